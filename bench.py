@@ -1,0 +1,251 @@
+"""Benchmark: DiffMC + DiffDMC forward+backward on a 256^3 fp32 grid, one MI355X.
+
+Workload (the N=256 forward+backward configuration): sdf = rounded box (half-extent 0.35,
+radius 0.1) + i.i.d. U(-0.05, 0.05) noise, deform = U(-0.3, 0.3) [N,N,N,3], upstream dL/dverts
+~ N(0,1) [V,3], all from numpy default_rng(seed) in that order, generated in float64 and cast to
+float32. One step = DiffMC forward+backward + DiffDMC forward+backward (quads) through the public
+API, each on its own copy of the inputs; two input sets alternate between steps so that the
+256 MB last-level cache cannot hold a step's inputs.
+
+Reported value: Gvoxel/s = (voxels extracted per step) / (seconds per step) / 1e9, with
+2 * N^3 voxels per step (one MC and one DMC extraction). Inputs are resident in HBM before the
+timed region. `roofline` times each engine kernel with HIP events on the stream it runs on and
+reports the slowest one against HBM peak using its algorithmic bytes (sdf / deform / gradient
+samples it must touch plus the vertices, faces and dense gradients it must write).
+`cpu_baseline` runs the OpenMP CPU oracle (the reference's per-cell algorithm) on the same
+workload on the host cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--n", type=int, default=256)
+    p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--kernel-reps", type=int, default=10)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-reps", type=int, default=3)
+    return p.parse_args()
+
+
+def make_inputs(n, seed, dev):
+    import torch
+
+    from diso_amd import workloads
+
+    sdf, deform, rng = workloads.noisy_round_box(n, seed=seed)
+    return (torch.tensor(sdf, device=dev), torch.tensor(deform, device=dev), rng, sdf, deform)
+
+
+def touched_vertices(sdf, iso=0.0):
+    """Grid vertices incident to at least one crossing edge of the padded grid."""
+    import torch
+    import torch.nn.functional as F
+
+    g = F.pad(sdf, (1, 1, 1, 1, 1, 1), value=iso + 1.0)
+    inside = g >= iso
+    t = torch.zeros_like(inside)
+    for ax in range(3):
+        a = inside.narrow(ax, 0, inside.shape[ax] - 1)
+        b = inside.narrow(ax, 1, inside.shape[ax] - 1)
+        c = a != b
+        t.narrow(ax, 0, inside.shape[ax] - 1).logical_or_(c)
+        t.narrow(ax, 1, inside.shape[ax] - 1).logical_or_(c)
+    return int(t[1:-1, 1:-1, 1:-1].sum().item())
+
+
+def time_kernels(ex, sdf, deform, flags, g, reps):
+    """Average duration (ms) of the count, emit and backward launches, HIP events on the
+    current stream (the one the library launches on)."""
+    import torch
+
+    from diso_amd import _lib
+
+    st = torch.cuda.current_stream()
+    res = {}
+    counted = ex.count(sdf, 0.0, flags)
+    verts, faces = ex.emit(sdf, deform, counted, flags)
+    gs, gd = torch.empty_like(sdf), torch.empty_like(deform)
+    L = _lib.lib()
+    ctl = ex.ctl_for(sdf.device, counted.dims, flags)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    sp = ctypes.c_void_p(st.cuda_stream)
+
+    def run_count():
+        rc = L.diso_count(ex.algo, ex.code, ptr(sdf), *counted.dims, 0.0, flags, ptr(counted.seg),
+                          counted.seg.numel(), ptr(ctl), ctl.numel(), None, sp)
+        _lib.check(rc, "count")
+
+    def run_emit():
+        ex.emit(sdf, deform, counted, flags)
+
+    def run_bwd():
+        ex.backward(sdf, deform, counted, flags, g, gs, gd)
+
+    for name, fn in (("count", run_count), ("emit", run_emit), ("backward", run_bwd)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / reps
+    return res, counted
+
+
+def algorithmic_bytes(kind, n, s, tv, nv, nf, k):
+    """Bytes a kernel must move (SURVEY byte model, per kernel):
+    count: every sample once; emit: samples + deform at touched vertices, vertices, int64 faces;
+    backward: samples + deform at touched vertices, upstream vertex gradients, dense sdf and deform
+    gradients."""
+    n3 = n ** 3
+    if kind == "count":
+        return s * n3
+    if kind == "emit":
+        return s * tv + 3 * s * tv + 3 * s * nv + 8 * k * nf
+    return s * tv + 3 * s * tv + 3 * s * nv + s * n3 + 3 * s * n3
+
+
+def cpu_baseline(sdf_np, deform_np, n, seed, reps):
+    from diso_amd import workloads
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    times = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        for algo in (O.MC, O.DMC):
+            ex = O.Extractor(algo, np.float32, threads=threads)
+            v, _ = ex.forward(sdf_np, deform_np, 0.0, True, return_quads=True)
+            g = np.random.default_rng(seed + 1).standard_normal(v.shape).astype(np.float32)
+            ex.backward(g)
+        dt = time.perf_counter() - t0
+        if r > 0:  # first run is a warm-up
+            times.append(dt)
+    med = float(np.median(times))
+    return {
+        "value": round(2 * n ** 3 / med / 1e9, 4),
+        "unit": "Gvoxel/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"full workload: MC+DMC fwd+bwd on the {n}^3 grid, median of {reps} runs after 1 warm-up "
+                  f"({med:.2f} s per step; OpenMP oracle, {threads} threads)",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    import diso_amd
+    from diso_amd import _lib, workloads
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    sets = [make_inputs(n, args.seed + i, dev) for i in range(2)]
+    mods = {"mc": diso_amd.DiffMC(torch.float32), "dmc": diso_amd.DiffDMC(torch.float32)}
+
+    # Upstream gradients, drawn after V is known (per input set and algorithm).
+    ups = []
+    for sdf, deform, rng, _, _ in sets:
+        d = {}
+        for name, mod in mods.items():
+            kw = {"return_quads": True} if name == "dmc" else {}
+            with torch.no_grad():
+                v, _ = mod(sdf, deform, 0.0, **kw)
+            d[name] = torch.tensor(workloads.upstream(rng, v.shape[0]), device=dev)
+        ups.append(d)
+
+    def step(i):
+        sdf0, def0 = sets[i % 2][0], sets[i % 2][1]
+        for name, mod in mods.items():
+            sdf = sdf0.detach().requires_grad_(True)
+            deform = def0.detach().requires_grad_(True)
+            kw = {"return_quads": True} if name == "dmc" else {}
+            v, f = mod(sdf, deform, 0.0, **kw)
+            torch.autograd.backward(v, grad_tensors=ups[i % 2][name])
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    per = []
+    st = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        step(i)
+        e1.record(st)
+        per.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = np.array([a.elapsed_time(b) for a, b in per])
+    ms_per_step = wall / args.steps * 1e3
+    value = 2 * n ** 3 / (ms_per_step / 1e3) / 1e9
+
+    # Per-kernel timing and roofline on input set 0.
+    sdf, deform = sets[0][0], sets[0][1]
+    tv = touched_vertices(sdf)
+    flags = _lib.FLAG_VIRTUAL_PAD | _lib.FLAG_SHIFT | _lib.FLAG_NORMALIZE | _lib.FLAG_INDEX64
+    kern = {}
+    for name, mod in mods.items():
+        t, counted = time_kernels(mod._ex, sdf, deform, flags, ups[0][name], args.kernel_reps)
+        k = 3 if name == "mc" else 4
+        for kind, ms in t.items():
+            b = algorithmic_bytes(kind, n, 4, tv, counted.n_verts, counted.n_faces, k)
+            kern[f"{name}_{kind}"] = {"ms": round(ms, 4), "bytes": int(b), "GBs": round(b / (ms / 1e3) / 1e9, 1)}
+        kern[f"{name}_counts"] = {"verts": counted.n_verts, "faces": counted.n_faces}
+    timed = {k: v for k, v in kern.items() if "ms" in v}
+    dom = max(timed, key=lambda k: timed[k]["ms"])
+    d = timed[dom]
+
+    out = {
+        "metric": "Gvoxel/s fwd+bwd DiffMC/DMC 256^3 fp32, batched vs plain; p50/p99 ms",
+        "value": round(value, 3),
+        "unit": "Gvoxel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "p50_ms": round(float(np.percentile(step_ms, 50)), 3),
+        "p99_ms": round(float(np.percentile(step_ms, 99)), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain DiffMC + DiffDMC(quads) fwd+bwd, rounded box + U(-0.05,0.05) noise, "
+                               f"deform U(-0.3,0.3), N={n}", "grid": n, "grids_per_step": 2,
+                   "mode": "plain per-grid API", "touched_vertices": tv},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(d["GBs"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "kernels": kern,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sets[0][3], sets[0][4], n, args.seed, args.cpu_reps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

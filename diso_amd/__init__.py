@@ -1,0 +1,143 @@
+"""diso_amd - differentiable iso-surface extraction (Marching Cubes and Dual Marching Cubes) on
+AMD Instinct MI355X, a drop-in for the reference package `diso` (diso/__init__.py:1-147).
+
+    from diso_amd import DiffMC, DiffDMC
+    verts, faces = DiffMC(dtype=torch.float32)(sdf, deform=None, isovalue=0.0, normalize=True)
+    verts, faces = DiffDMC(dtype=torch.float32)(sdf, deform=None, isovalue=0.0,
+                                                return_quads=False, normalize=True)
+
+Same arguments, outputs, dtypes and ordering as the reference; backward flows to `sdf` and
+`deform` through torch.autograd. What the reference does with several ATen calls around its
+engine (early-out test, F.pad, `-1`, normalisation, `.long()`) is done inside the HIP passes
+here: padding is virtual, the epilogue is written by the emit kernel and the early-out test is a
+by-product of the counting pass, so a forward costs one host synchronisation (to size the
+outputs) and a backward none.
+"""
+import torch
+import torch.nn.functional as F
+from torch import nn
+
+from . import _lib
+from .engine import Extractor
+
+__all__ = ["DiffMC", "DiffDMC"]
+__version__ = "0.1.0"
+
+
+def _check_inputs(ex, grid, deform):
+    dims = tuple(grid.shape)
+    if len(dims) != 3:
+        raise ValueError(f"grid must have shape [X, Y, Z], got {list(dims)}")
+    ex.check_grid(grid, "grid")
+    if deform is not None:
+        ex.check_grid(deform, "deformation")
+        if tuple(deform.shape) != dims + (3,):
+            raise ValueError(f"deform must have shape {list(dims) + [3]}, got {list(deform.shape)}")
+        if deform.device != grid.device:
+            raise RuntimeError("grid and deform must be on the same device")
+
+
+class _Extract(torch.autograd.Function):
+    """Emission of an already counted grid, with the gather backward of the HIP engine.
+
+    The reference recomputes its whole forward inside backward (diso/__init__.py:32,86) because
+    its engine keeps per-call state; here the per-call state (segment bases) is kept by autograd
+    in `counted`, so the backward is a single kernel."""
+
+    @staticmethod
+    def forward(ctx, grid, deform, ex, counted, flags):
+        verts, faces = ex.emit(grid, deform, counted, flags)
+        ctx.ex, ctx.counted, ctx.flags = ex, counted, flags
+        ctx.save_for_backward(grid, deform)
+        ctx.mark_non_differentiable(faces)
+        return verts, faces
+
+    @staticmethod
+    def backward(ctx, adj_verts, adj_faces):
+        grid, deform = ctx.saved_tensors
+        grad_grid = torch.empty_like(grid)
+        grad_deform = None if deform is None else torch.empty_like(deform)
+        with torch.cuda.device(grid.device):
+            ctx.ex.backward(grid, deform, ctx.counted, ctx.flags, adj_verts, grad_grid, grad_deform)
+        return grad_grid, grad_deform, None, None, None
+
+
+def _run(ex, grid, deform, isovalue, normalize, compat, empty_k):
+    _check_inputs(ex, grid, deform)
+    grid = grid.contiguous()
+    deform = None if deform is None else deform.contiguous()
+    iso = float(isovalue)
+    flags = _lib.FLAG_VIRTUAL_PAD | _lib.FLAG_SHIFT | _lib.FLAG_INDEX64
+    if normalize:
+        flags |= _lib.FLAG_NORMALIZE
+    if compat:
+        flags |= _lib.FLAG_REFERENCE_COMPAT
+    with torch.cuda.device(grid.device):
+        counted = ex.count(grid, iso, flags)
+        if counted.has_nan:
+            raise RuntimeError("diso_amd: sdf grid contains NaN values")
+        if counted.empty_grid:
+            # diso/__init__.py:49-50,103-104: fresh tensors, int32 faces, no autograd graph.
+            return (torch.zeros((0, 3), dtype=ex.dtype, device=grid.device),
+                    torch.zeros((0, empty_k), dtype=torch.int32, device=grid.device), True)
+        verts, faces = _Extract.apply(grid, deform, ex, counted, flags)
+        return verts, faces, False
+
+
+class DiffMC(nn.Module):
+    """Differentiable Marching Cubes (diso/__init__.py:9-61)."""
+
+    def __init__(self, dtype=torch.float32):
+        super().__init__()
+        self.dtype = dtype
+        self._ex = Extractor(_lib.MC, dtype)
+
+    def forward(self, grid, deform=None, isovalue=0.0, normalize=True):
+        verts, faces, _ = _run(self._ex, grid, deform, isovalue, normalize, False, 3)
+        return verts, faces
+
+
+def split_quads(verts, quads):
+    """Cuts each quad into two triangles along the diagonal that keeps the smallest angle
+    largest, exactly as diso/__init__.py:117-147: the score of a cut is the largest corner cosine
+    over its two triangles (F.normalize with eps 1e-12); cut (0,1,3)+(1,2,3) is taken when its
+    score is strictly smaller than that of (0,1,2)+(0,2,3). All quads taking the first cut come
+    first, in quad order, then the others."""
+
+    def worst_cosine(cut):
+        per_tri = []
+        for a, b, c in cut:
+            v0, v1, v2 = verts[quads[:, a]], verts[quads[:, b]], verts[quads[:, c]]
+            c1 = (F.normalize(v1 - v0, dim=-1) * F.normalize(v2 - v0, dim=-1)).sum(-1)
+            c2 = (F.normalize(v2 - v1, dim=-1) * F.normalize(v0 - v1, dim=-1)).sum(-1)
+            c3 = (F.normalize(v0 - v2, dim=-1) * F.normalize(v1 - v2, dim=-1)).sum(-1)
+            per_tri.append(torch.max(torch.stack([c1, c2, c3], dim=-1), dim=-1)[0])
+        return torch.max(torch.stack(per_tri, dim=-1), dim=1)[0]
+
+    with torch.no_grad():
+        s1 = worst_cosine(((0, 1, 3), (1, 2, 3)))
+        s2 = worst_cosine(((0, 1, 2), (0, 2, 3)))
+        first = s1 < s2
+        f1 = quads[first][:, [0, 1, 3, 1, 2, 3]].view(-1, 3)
+        f2 = quads[~first][:, [0, 1, 2, 0, 2, 3]].view(-1, 3)
+        return torch.cat([f1, f2], dim=0)
+
+
+class DiffDMC(nn.Module):
+    """Differentiable Dual Marching Cubes (diso/__init__.py:63-147).
+
+    reference_compat=True reproduces the reference backward's read of each cell's first patch
+    gradient for all of the cell's patches (src/cudualmc.cu:990); the default computes the true
+    gradient of the forward (the two agree whenever every cell has at most one patch)."""
+
+    def __init__(self, dtype=torch.float32, reference_compat=False):
+        super().__init__()
+        self.dtype = dtype
+        self.reference_compat = reference_compat
+        self._ex = Extractor(_lib.DMC, dtype)
+
+    def forward(self, grid, deform=None, isovalue=0.0, return_quads=False, normalize=True):
+        verts, quads, empty = _run(self._ex, grid, deform, isovalue, normalize, self.reference_compat, 4)
+        if return_quads or empty:
+            return verts, quads
+        return verts, split_quads(verts, quads)

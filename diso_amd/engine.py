@@ -1,0 +1,121 @@
+"""Thin host-side driver of libdiso_hip.so for torch tensors.
+
+`Extractor` is the counterpart of the reference's engine objects (CuMC, src/cumc.h:70-139;
+CUDualMC, src/cudualmc.h:70-150): it owns the scan control workspace of each device it has run
+on (zero-filled once, grown on demand) and runs the count / emit / backward passes on the
+current torch stream. Per-call state (the segment bases the backward needs) is returned to the
+caller instead of being kept inside the object, so one extractor can serve many grids whose
+backward runs later (autograd, activation checkpointing).
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+_DTYPES = {torch.float32: _lib.F32, torch.float64: _lib.F64}
+
+
+class Counted:
+    """Result of the counting pass: segment bases on the device and the totals on the host."""
+
+    __slots__ = ("seg", "n_verts", "n_faces", "flags", "dims", "iso", "flags_in")
+
+    def __init__(self, seg, totals, dims, iso, flags_in):
+        self.seg = seg
+        self.n_verts = int(totals.n_verts)
+        self.n_faces = int(totals.n_faces)
+        self.flags = int(totals.flags)
+        self.dims = dims
+        self.iso = iso
+        self.flags_in = flags_in
+
+    @property
+    def empty_grid(self):
+        """The reference wrapper's early-out: min(sdf) >= iso or max(sdf) <= iso
+        (diso/__init__.py:49,103)."""
+        both = _lib.TOTAL_HAS_BELOW | _lib.TOTAL_HAS_ABOVE
+        return (self.flags & both) != both
+
+    @property
+    def has_nan(self):
+        return bool(self.flags & _lib.TOTAL_HAS_NAN)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class Extractor:
+    def __init__(self, algo, dtype):
+        if dtype not in _DTYPES:
+            raise ValueError(f"diso_amd: dtype must be torch.float32 or torch.float64, got {dtype}")
+        self.algo = algo
+        self.dtype = dtype
+        self.code = _DTYPES[dtype]
+        self._ctl = {}
+        _lib.lib()  # fail loudly now if the HIP library is not there
+
+    def check_grid(self, grid, name="grid"):
+        if not isinstance(grid, torch.Tensor):
+            raise TypeError(f"{name} must be a torch.Tensor")
+        if grid.device.type != "cuda":
+            raise RuntimeError(f"{name} must be a CUDA tensor")
+        if grid.dtype != self.dtype:
+            raise RuntimeError(f"{name} type must match the {'mc' if self.algo == _lib.MC else 'dmc'} class")
+
+    def ctl_for(self, device, dims, flags):
+        need = _lib.lib().diso_ctl_bytes(*dims, flags)
+        if need < 0:
+            _lib.check(need, "workspace sizing")
+        key = device.index
+        ctl = self._ctl.get(key)
+        if ctl is None or ctl.numel() < need:
+            # Grown buffers start zero-filled; the scan keeps them consistent from then on.
+            ctl = torch.zeros(max(need, 1 << 16) * 5 // 4, dtype=torch.uint8, device=device)
+            self._ctl[key] = ctl
+        return ctl
+
+    def count(self, grid, iso, flags):
+        dims = tuple(int(d) for d in grid.shape)
+        L = _lib.lib()
+        nseg = L.diso_seg_bytes(*dims, flags)
+        if nseg < 0:
+            _lib.check(nseg, "workspace sizing")
+        seg = torch.empty(nseg, dtype=torch.uint8, device=grid.device)
+        ctl = self.ctl_for(grid.device, dims, flags)
+        tot = _lib.Totals()
+        rc = L.diso_count(self.algo, self.code, _ptr(grid), *dims, float(iso), flags, _ptr(seg), nseg,
+                          _ptr(ctl), ctl.numel(), ctypes.byref(tot), _stream(grid.device))
+        _lib.check(rc, "count pass")
+        return Counted(seg, tot, dims, float(iso), flags)
+
+    def emit(self, grid, deform, counted, flags):
+        """Vertices [V,3] and faces ([F,3] MC triangles / [Q,4] DMC quads) of a counted grid."""
+        L = _lib.lib()
+        k = 3 if self.algo == _lib.MC else 4
+        idx = torch.int64 if flags & _lib.FLAG_INDEX64 else torch.int32
+        verts = torch.empty((counted.n_verts, 3), dtype=self.dtype, device=grid.device)
+        faces = torch.empty((counted.n_faces, k), dtype=idx, device=grid.device)
+        rc = L.diso_emit(self.algo, self.code, _ptr(grid), _ptr(deform), *counted.dims, counted.iso, flags,
+                         _ptr(counted.seg), counted.seg.numel(), _ptr(verts), counted.n_verts, _ptr(faces),
+                         counted.n_faces, _stream(grid.device))
+        _lib.check(rc, "emit pass")
+        return verts, faces
+
+    def backward(self, grid, deform, counted, flags, adj_verts, adj_grid, adj_deform):
+        """Writes (or, with FLAG_ACCUMULATE, adds) d(sum(adj_verts*verts))/d(grid, deform)."""
+        L = _lib.lib()
+        adj_verts = adj_verts.contiguous()
+        if adj_verts.dtype != self.dtype:
+            raise RuntimeError("adj_verts type must match the extractor's dtype")
+        if adj_verts.shape != (counted.n_verts, 3):
+            raise ValueError(f"adj_verts must have shape ({counted.n_verts}, 3), got {tuple(adj_verts.shape)}")
+        rc = L.diso_backward(self.algo, self.code, _ptr(grid), _ptr(deform), *counted.dims, counted.iso, flags,
+                             _ptr(counted.seg), counted.seg.numel(), _ptr(adj_verts), counted.n_verts,
+                             _ptr(adj_grid), _ptr(adj_deform), _stream(grid.device))
+        _lib.check(rc, "backward pass")

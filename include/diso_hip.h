@@ -1,0 +1,135 @@
+/* diso_hip.h - C ABI of the MI355X (gfx950) DiffMC / DiffDMC engine (libdiso_hip.so).
+ *
+ * This ABI replaces the reference's pybind11 classes _C.CUMCFloat / CUMCDouble / CUDMCFloat /
+ * CUDMCDouble (src/pybind.cpp:16-395,419-447) and the engine structs behind them (CuMC,
+ * src/cumc.h:70-139; CUDualMC, src/cudualmc.h:70-150). It takes plain device pointers, sizes and
+ * a HIP stream; it never allocates device memory and never synchronises the device. The caller
+ * owns every buffer (two workspaces described below, inputs and outputs).
+ *
+ * Conventions shared by all entry points
+ *  - algo: DISO_MC (Marching Cubes) or DISO_DMC (Dual Marching Cubes).
+ *  - dtype: DISO_F32 or DISO_F64; sdf, deform, vertices and gradients all use it.
+ *  - sdf is a contiguous [X,Y,Z] grid (z fastest); deform, when not NULL, is contiguous
+ *    [X,Y,Z,3]. With DISO_FLAG_VIRTUAL_PAD the grid is the user's unpadded grid and the engine
+ *    behaves exactly as if it had been padded by one sample of value (iso + 1) (deform by 0), as
+ *    the reference's Python wrapper does (diso/__init__.py:52-54) before calling its engine;
+ *    without it the grid is taken as given, like the reference's _C classes.
+ *  - iso is the Python float; it is rounded to dtype for comparisons and the pad value is
+ *    (dtype)(iso + 1.0), matching pybind's conversion and F.pad's fill.
+ *  - Every entry point returns 0 on success or a negative DISO_E* code; diso_last_error() then
+ *    holds a message (thread-local). The Python layer raises RuntimeError, the reference's
+ *    exception type for its input checks (src/pybind.cpp:5-11,57-60).
+ *  - stream is a hipStream_t (NULL = the default stream). All work is stream-ordered.
+ */
+#ifndef DISO_HIP_H
+#define DISO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { DISO_MC = 0, DISO_DMC = 1 };
+enum { DISO_F32 = 0, DISO_F64 = 1 };
+
+enum {
+    DISO_FLAG_VIRTUAL_PAD = 1 << 0, /* input is unpadded; pad with iso+1 / 0 on the fly          */
+    DISO_FLAG_SHIFT = 1 << 1,       /* emit vertices minus 1 (diso/__init__.py:56,110)           */
+    DISO_FLAG_NORMALIZE = 1 << 2,   /* then divide by (dims-1) per axis (diso/__init__.py:57-60) */
+    DISO_FLAG_INDEX64 = 1 << 3,     /* faces as int64 (the wrapper's .long()); else int32        */
+    DISO_FLAG_REFERENCE_COMPAT = 1 << 4, /* DMC backward reads each cell's first patch gradient
+                                            for all its patches, as src/cudualmc.cu:990 does    */
+    DISO_FLAG_ACCUMULATE = 1 << 5   /* backward adds into the gradient buffers instead of
+                                       overwriting them                                        */
+};
+
+enum {
+    DISO_OK = 0,
+    DISO_E_INVALID = -1,  /* bad argument                                                   */
+    DISO_E_HIP = -2,      /* HIP runtime error                                              */
+    DISO_E_CAPACITY = -3, /* a workspace or output buffer is too small                      */
+    DISO_E_TIMEOUT = -4   /* the single-pass scan did not complete (never expected)         */
+};
+
+/* Status bits of diso_totals_t.flags. */
+enum {
+    DISO_TOTAL_HAS_BELOW = 1 << 0, /* some sample < iso  (i.e. not min >= iso)               */
+    DISO_TOTAL_HAS_ABOVE = 1 << 1, /* some sample > iso  (i.e. not max <= iso)               */
+    DISO_TOTAL_HAS_NAN = 1 << 2,   /* some sample is NaN                                      */
+    DISO_TOTAL_SCAN_TIMEOUT = 1 << 3
+};
+
+/* Result of the counting pass, read back to size the outputs.
+ *  MC : n_verts = vertices (= crossing edges), n_faces = triangles.
+ *  DMC: n_verts = vertices (= patches),        n_faces = quads (= crossing edges).
+ * The wrapper's empty-grid early-out (diso/__init__.py:49,103) applies when
+ * (flags & (HAS_BELOW|HAS_ABOVE)) != (HAS_BELOW|HAS_ABOVE). */
+typedef struct {
+    int64_t n_verts;
+    int64_t n_faces;
+    int64_t flags;
+    int64_t n_crossing;
+} diso_totals_t;
+
+/* Size in bytes of the per-call segment workspace ("seg") for a grid of dims (X,Y,Z): the
+ * per-segment vertex/face bases written by diso_count and read by diso_emit / diso_backward.
+ * It needs no initialisation. Keep it (with the same inputs) to run diso_backward later; that
+ * is the state the reference keeps inside CuMC between forward and backward (src/cumc.h:73-99). */
+int64_t diso_seg_bytes(int64_t X, int64_t Y, int64_t Z, int flags);
+
+/* Size in bytes of the control workspace ("ctl") that serves a grid of dims (X,Y,Z) or any
+ * smaller one. It must be zero-filled once when allocated and is then reused across calls; calls
+ * that share a ctl must not overlap in time (it holds the single-pass scan's tile states). */
+int64_t diso_ctl_bytes(int64_t X, int64_t Y, int64_t Z, int flags);
+
+/* Pass 1: classify every cell, count vertices and faces per segment and scan the counts in one
+ * single-pass launch. Replaces count_used_cells_kernel, index_used_cells_kernel,
+ * count_cell_mc_verts_kernel, count_cell_mc_tris_kernel / count_cell_patches_kernel and the three
+ * CUB scans of CuMC::forward (src/cumc.cu:651-723) and CUDualMC::forward
+ * (src/cudualmc.cu:1058-1121). If totals_host is not NULL the totals are copied back and the
+ * stream is synchronised (the one host sync the plain API needs to size its outputs). */
+int diso_count(int algo, int dtype, const void *sdf, int64_t X, int64_t Y, int64_t Z, double iso,
+               int flags, void *seg, int64_t seg_bytes, void *ctl, int64_t ctl_bytes,
+               diso_totals_t *totals_host, void *stream);
+
+/* Pass 2: write vertices [n_verts,3] and faces ([n_faces,3] MC triangles, [n_faces,4] DMC quads)
+ * in the reference's order. Replaces create_cell_mc_verts_kernel + create_cell_mc_tris_kernel
+ * (src/cumc.cu:370-410,564-649) and create_dmc_verts_kernel + create_quads_kernel
+ * (src/cudualmc.cu:907-955,1027-1056), plus the wrapper's shift / normalise / int64 epilogue
+ * (diso/__init__.py:56-61,110-116) when the corresponding flags are set. Writes never exceed
+ * vert_capacity / face_capacity (rows); the call fails with DISO_E_CAPACITY if the counts from
+ * diso_count do not fit (checked on the device; the outputs are then not written). */
+int diso_emit(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y,
+              int64_t Z, double iso, int flags, const void *seg, int64_t seg_bytes, void *verts,
+              int64_t vert_capacity, void *faces, int64_t face_capacity, void *stream);
+
+/* Backward: gradients of sum(adj_verts * verts) with respect to sdf [X,Y,Z] and deform
+ * [X,Y,Z,3] (adj_deform may be NULL). One thread per grid vertex gathers from its at most six
+ * incident crossing edges, so there are no atomics and results are deterministic. Replaces
+ * adj_create_cell_mc_verts_kernel (src/cumc.cu:474-512,734-743) and adj_create_dmc_verts_kernel
+ * (src/cudualmc.cu:957-1005,1130-1138), plus the chain rule through the wrapper's normalise and
+ * the slicing of F.pad's backward. adj_verts is the gradient with respect to the vertices as
+ * emitted with the same flags. Gradients are written for every grid vertex (zero where no
+ * crossing edge touches it) unless DISO_FLAG_ACCUMULATE is set. `seg` must hold the result of a
+ * diso_count on the same inputs. */
+int diso_backward(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y,
+                  int64_t Z, double iso, int flags, const void *seg, int64_t seg_bytes,
+                  const void *adj_verts, int64_t n_verts, void *adj_sdf, void *adj_deform,
+                  void *stream);
+
+/* Reads the totals of the last diso_count on `seg` (blocking copy on `stream`). */
+int diso_read_totals(const void *seg, int64_t seg_bytes, int64_t X, int64_t Y, int64_t Z,
+                     int flags, diso_totals_t *totals_host, void *stream);
+
+/* Thread-local message for the last failing call on this thread ("" if none). */
+const char *diso_last_error(void);
+
+/* Library version string. */
+const char *diso_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DISO_HIP_H */

@@ -1,0 +1,44 @@
+"""Shared inputs and comparisons for the tests."""
+import numpy as np
+
+from diso_amd import workloads
+
+
+def field(dims, kind, seed):
+    """A [X,Y,Z] float64 sdf of the given family on an anisotropic grid."""
+    rng = np.random.default_rng(seed)
+    X, Y, Z = dims
+    axes = [np.linspace(-0.5, 0.5, n) for n in dims]
+    p = np.stack(np.meshgrid(*axes, indexing="ij"), -1)
+    if kind == "sphere":
+        return np.linalg.norm(p, axis=-1) - 0.37
+    if kind == "noisy_box":
+        q = np.abs(p) - 0.3 + 0.1
+        f = np.linalg.norm(np.maximum(q, 0), axis=-1) + np.minimum(q.max(-1), 0) - 0.1
+        return f + rng.uniform(-0.05, 0.05, size=dims)
+    if kind == "noise":
+        return rng.uniform(-1.0, 1.0, size=dims)
+    if kind == "blobs":
+        f = np.zeros(dims)
+        for _ in range(6):
+            c = rng.uniform(-0.3, 0.3, 3)
+            s = rng.uniform(0.05, 0.15)
+            f += np.exp(-((p - c) ** 2).sum(-1) / (2 * s * s))
+        return f - 0.5
+    raise ValueError(kind)
+
+
+def inputs(dims, kind, seed, dtype, with_deform, deform_amp=0.3):
+    rng = np.random.default_rng(seed + 1000)
+    sdf = field(dims, kind, seed).astype(dtype)
+    deform = rng.uniform(-deform_amp, deform_amp, size=tuple(dims) + (3,)).astype(dtype) if with_deform else None
+    return sdf, deform
+
+
+def grad_close(got, want, rel):
+    """|got - want| <= rel * max(|want|) elementwise (gradients accumulate the same terms in a
+    different order than the oracle, so they are compared against the field's scale)."""
+    scale = max(float(np.abs(want).max()), 1e-30)
+    err = float(np.abs(got.astype(np.float64) - want.astype(np.float64)).max())
+    assert err <= rel * scale, f"max abs err {err:.3e} > {rel:.1e} * scale {scale:.3e}"
+    return err / scale

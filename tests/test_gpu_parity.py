@@ -1,0 +1,173 @@
+"""HIP engine against the CPU oracle on the same seeded inputs (needs a GPU).
+
+Topology (faces / quads, their order and the vertex count) and fp32/fp64 vertex positions must
+be bit-identical to the oracle, which restates the reference engine operation by operation.
+Gradients are compared against the oracle's sequential scatter within a tolerance relative to
+the gradient's scale: 2e-5 (fp32) / 1e-12 (fp64), since the gather sums the same terms in a
+different order.
+"""
+import numpy as np
+import pytest
+import torch
+
+import diso_amd
+from diso_amd import _C
+from oracle import oracle as O
+from tests.helpers import grad_close, inputs
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+TOL = {np.float32: 2e-5, np.float64: 1e-12}
+TDT = {np.float32: torch.float32, np.float64: torch.float64}
+
+CASES = [
+    ((24, 24, 24), "noisy_box"),
+    ((17, 33, 70), "noise"),
+    ((5, 3, 66), "noise"),
+    ((40, 9, 130), "blobs"),
+    ((64, 64, 64), "sphere"),
+    ((2, 2, 2), "noise"),
+    ((1, 7, 3), "noise"),  # X=1: normalisation divides by X-1 = 0, so run unnormalised
+]
+
+
+def _module(algo, dtype, compat=False):
+    return diso_amd.DiffMC(TDT[dtype]) if algo == O.MC else diso_amd.DiffDMC(TDT[dtype], reference_compat=compat)
+
+
+def _run_both(algo, dtype, dims, kind, with_deform, seed=0, normalize=True, iso=0.0, compat=False):
+    sdf_np, def_np = inputs(dims, kind, seed, dtype, with_deform)
+    print(f"case algo={algo} dtype={np.dtype(dtype).name} dims={dims} kind={kind} deform={with_deform}", flush=True)
+    ref = O.Extractor(algo, dtype)
+    rv, rf = ref.forward(sdf_np, def_np, iso, normalize, return_quads=True)
+    sdf = torch.tensor(sdf_np, device=DEV, requires_grad=True)
+    deform = None if def_np is None else torch.tensor(def_np, device=DEV, requires_grad=True)
+    kw = {"return_quads": True} if algo == O.DMC else {}
+    verts, faces = _module(algo, dtype, compat)(sdf, deform, iso, normalize=normalize, **kw)
+    return ref, rv, rf, sdf, deform, verts, faces
+
+
+@pytest.mark.parametrize("algo", [O.MC, O.DMC])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("with_deform", [False, True])
+@pytest.mark.parametrize("dims,kind", CASES)
+def test_forward_backward_match_oracle(algo, dtype, with_deform, dims, kind):
+    ref, rv, rf, sdf, deform, verts, faces = _run_both(algo, dtype, dims, kind, with_deform,
+                                                       normalize=min(dims) > 1)
+    if ref.empty:
+        assert verts.shape == (0, 3) and faces.shape[0] == 0
+        return
+    assert faces.dtype == torch.int64
+    assert tuple(faces.shape) == rf.shape
+    assert np.array_equal(faces.cpu().numpy(), rf)
+    assert np.array_equal(verts.detach().cpu().numpy(), rv)
+    rng = np.random.default_rng(7)
+    g = rng.standard_normal(rv.shape).astype(dtype)
+    (verts * torch.tensor(g, device=DEV)).sum().backward()
+    gs, gd = ref.backward(g)
+    grad_close(sdf.grad.cpu().numpy(), gs, TOL[dtype])
+    if deform is not None:
+        grad_close(deform.grad.cpu().numpy(), gd, TOL[dtype])
+
+
+@pytest.mark.parametrize("algo", [O.MC, O.DMC])
+def test_unnormalized_and_isovalue(algo):
+    ref, rv, rf, sdf, deform, verts, faces = _run_both(algo, np.float32, (20, 31, 67), "blobs", True,
+                                                       normalize=False, iso=0.1)
+    assert np.array_equal(faces.cpu().numpy(), rf)
+    assert np.array_equal(verts.detach().cpu().numpy(), rv)
+
+
+def test_dmc_reference_compat_backward():
+    ref, rv, rf, sdf, deform, verts, faces = _run_both(O.DMC, np.float64, (19, 22, 70), "noise", True,
+                                                       compat=True)
+    g = np.random.default_rng(3).standard_normal(rv.shape)
+    (verts * torch.tensor(g, device=DEV)).sum().backward()
+    gs, gd = ref.backward(g, reference_compat=True)
+    grad_close(sdf.grad.cpu().numpy(), gs, 1e-12)
+    grad_close(deform.grad.cpu().numpy(), gd, 1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_dmc_triangles_match_oracle_split(dtype):
+    sdf_np, def_np = inputs((33, 30, 41), "noisy_box", 5, dtype, True)
+    ref = O.Extractor(O.DMC, dtype)
+    rv, rt = ref.forward(sdf_np, def_np, 0.0, True, return_quads=False)
+    _, rq = O.Extractor(O.DMC, dtype).forward(sdf_np, def_np, 0.0, True, return_quads=True)
+    _, s1, s2 = O.split_quads(rv, rq)
+    verts, tris = diso_amd.DiffDMC(TDT[dtype])(torch.tensor(sdf_np, device=DEV), torch.tensor(def_np, device=DEV))
+    tris = tris.cpu().numpy()
+    assert np.array_equal(verts.cpu().numpy(), rv)
+    assert tris.shape == rt.shape
+    # The cut choice compares float scores; quads whose two scores tie within rounding may be cut
+    # either way (and then move between the two halves of the output), so compare per-quad cuts.
+    first = s1 < s2
+    tie = np.abs(s1 - s2) <= 4 * np.finfo(dtype).eps
+    n1 = int(first.sum())
+    got_first = np.zeros(len(rq), bool)
+    got = {tuple(q) for q in tris[: 2 * n1].reshape(-1, 6)} if n1 else set()
+    for i, q in enumerate(rq):
+        got_first[i] = tuple(q[[0, 1, 3, 1, 2, 3]]) in got
+    assert np.all((got_first == first) | tie)
+    if not tie.any():
+        assert np.array_equal(tris, rt)
+
+
+@pytest.mark.parametrize("cls,algo,dtype", [(_C.CUMCFloat, O.MC, np.float32), (_C.CUMCDouble, O.MC, np.float64),
+                                            (_C.CUDMCFloat, O.DMC, np.float32), (_C.CUDMCDouble, O.DMC, np.float64)])
+def test_binding_classes_match_engine(cls, algo, dtype):
+    sdf_np, def_np = inputs((21, 18, 69), "noise", 11, dtype, True)
+    g_np, d_np = O.pad(sdf_np, def_np, 0.0)
+    eng = O.Engine(algo, dtype)
+    rv, rf, st = eng.forward(g_np, d_np, 0.0)
+    assert st == 0
+    b = cls()
+    grid = torch.tensor(g_np, device=DEV)
+    deform = torch.tensor(d_np, device=DEV)
+    verts, faces = b.forward(grid, deform, 0.0)
+    assert faces.dtype == torch.int32
+    assert np.array_equal(faces.cpu().numpy().astype(np.int64), rf)
+    assert np.array_equal(verts.cpu().numpy(), rv)
+    adj = np.random.default_rng(2).standard_normal(rv.shape).astype(dtype)
+    ag = torch.zeros_like(grid)
+    ad = torch.zeros_like(deform)
+    b.backward(grid, deform, 0.0, torch.tensor(adj, device=DEV), ag, ad)
+    gs, gd = eng.backward(g_np, d_np, 0.0, adj, reference_compat=(algo == O.DMC))
+    grad_close(ag.cpu().numpy(), gs, TOL[dtype])
+    grad_close(ad.cpu().numpy(), gd, TOL[dtype])
+    # no-deform overloads
+    verts2, faces2 = b.forward(grid, 0.0)
+    rv2, rf2, _ = O.Engine(algo, dtype).forward(g_np, None, 0.0)
+    assert np.array_equal(faces2.cpu().numpy().astype(np.int64), rf2)
+    assert np.array_equal(verts2.cpu().numpy(), rv2)
+
+
+def test_early_out_and_nan():
+    mc = diso_amd.DiffMC()
+    g = torch.full((8, 8, 8), 0.5, device=DEV)
+    v, f = mc(g)
+    assert v.shape == (0, 3) and f.shape == (0, 3) and f.dtype == torch.int32
+    g[3, 3, 3] = 0.0  # max == iso... min == iso: min >= iso holds -> empty
+    v, f = mc(g, isovalue=0.0)
+    assert v.shape == (0, 3)
+    v, f = diso_amd.DiffDMC()(torch.full((8, 8, 8), -1.0, device=DEV))
+    assert v.shape == (0, 3) and f.shape == (0, 4) and f.dtype == torch.int32
+    g = torch.randn(8, 8, 8, device=DEV)
+    g[1, 2, 3] = float("nan")
+    with pytest.raises(RuntimeError):
+        mc(g)
+
+
+def test_deterministic_and_repeatable():
+    sdf_np, def_np = inputs((48, 40, 72), "noise", 4, np.float32, True)
+    outs = []
+    for _ in range(2):
+        for mod in (diso_amd.DiffMC(), diso_amd.DiffDMC()):
+            sdf = torch.tensor(sdf_np, device=DEV, requires_grad=True)
+            deform = torch.tensor(def_np, device=DEV, requires_grad=True)
+            v, f = mod(sdf, deform)
+            (v * v).sum().backward()
+            outs.append((v.detach().cpu().numpy(), f.cpu().numpy(), sdf.grad.cpu().numpy(), deform.grad.cpu().numpy()))
+    for a, b in zip(outs[:2], outs[2:]):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
