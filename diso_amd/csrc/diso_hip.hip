@@ -37,8 +37,6 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
-constexpr int kSegsPerWaveCount = 2;  // count kernel: segments per wave
-constexpr int kSegsPerBlockCount = kWavesPerBlock * kSegsPerWaveCount;
 constexpr uint32_t kSpinLimit = 1u << 22;  // bounded look-back wait (~seconds); never expected
 constexpr int kCtlHeader = 64;             // bytes
 
@@ -280,32 +278,40 @@ __device__ __forceinline__ int64_t seg_index(const Geo &g, int px, int py, int c
 }
 
 // ------------------------------------------------------------------------------------------
-// Pass 1: classify, count, scan (single pass, decoupled look-back)
+// Pass 1: classify, count, scan (single pass)
 //
-// Tile state words (uint64): [63:62] status (1 = block aggregate, 2 = inclusive prefix),
-// [61:59] flags (f words only), [58:0] value. Each word is self-contained and accessed with
-// agent-scope atomics, so no fences are needed across XCDs. Two tile buffers alternate between
-// calls; each call clears the one the next call will use (up to the extent that call used),
-// and the last block flips the parity and resets the ticket, so no separate init launch exists.
+// The grid of segments is split into G contiguous ranges, one per block (G ~ 4 blocks per CU, so
+// all blocks are resident). A block counts its segments (writing each segment's counts into
+// `seg`), publishes its aggregate as one self-contained 64-bit word, reads the aggregates of all
+// its predecessors in parallel (a look-back over the whole window at once: no chain of inclusive
+// prefixes, so the wait is one memory latency rather than one per predecessor), and finally scans
+// its own segment counts into segment bases. Aggregate words are accessed with agent-scope
+// atomics only, so they are coherent across the XCDs' L2s without fences. Two word buffers
+// alternate between calls; each call clears the one the next call uses (up to the extent that
+// was written), and the last block flips the parity, so no initialisation launch is needed.
+//
+// Aggregate word: [63:62] status (1 = published), [61:59] sample flags, [58:29] face / patch
+// count, [28:0] vertex / quad count.
 
 struct CtlHeader {
-    uint32_t ticket;
     uint32_t parity;
-    uint32_t dirty[2];  // tile entries written in each buffer since it was last cleared
-    uint32_t pad[12];
+    uint32_t dirty[2];  // words written in each buffer since it was last cleared
+    uint32_t pad[13];
 };
 
-constexpr uint64_t kStA = 1ull << 62, kStP = 2ull << 62;
-constexpr uint64_t kValMask = (1ull << 59) - 1;
-__device__ __forceinline__ uint64_t st_of(uint64_t w) { return w >> 62; }
-__device__ __forceinline__ int64_t val_of(uint64_t w) { return (int64_t)(w & kValMask); }
-__device__ __forceinline__ uint32_t fl_of(uint64_t w) { return (uint32_t)((w >> 59) & 7); }
+constexpr int kCountWaves = 8;
+constexpr int kCountBlock = kCountWaves * kWave;
+constexpr int64_t kMaxCountBlocks = 8192;
+constexpr int kCountBlocksPerCU = 4;
 
 __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_relaxed(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t pack_agg(int64_t v, int64_t f, uint32_t fl) {
+    return (1ull << 62) | ((uint64_t)fl << 59) | ((uint64_t)f << 29) | (uint64_t)v;
 }
 
 // Per-position counts for pass 1: nv = crossing edges owned (MC vertices / DMC quads);
@@ -335,170 +341,168 @@ __device__ __forceinline__ void count_position(const In<S> &in, const uint64_t *
     }
 }
 
+__device__ __forceinline__ int64_t wave_incl64(int64_t v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int64_t u = __shfl_up(v, o, kWave);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
 template <typename S, int ALGO>
-__global__ __launch_bounds__(kBlock) void count_kernel(In<S> in, SegBase *seg, uint8_t *ctl,
-                                                       int64_t cap, int64_t nblocks) {
+__global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *seg, uint8_t *ctl,
+                                                            int64_t nblocks) {
     __shared__ uint64_t s_case[256];
-    __shared__ int64_t s_wv[kWavesPerBlock], s_wf[kWavesPerBlock];
-    __shared__ uint32_t s_wfl[kWavesPerBlock];
-    __shared__ int64_t s_bid;
+    __shared__ int64_t s_v[kCountWaves], s_f[kCountWaves];
+    __shared__ uint32_t s_fl[kCountWaves];
     __shared__ uint32_t s_parity, s_dirty_other;
-    __shared__ int64_t s_ev, s_ef;
 
     CtlHeader *hdr = reinterpret_cast<CtlHeader *>(ctl);
-    uint64_t *tiles = reinterpret_cast<uint64_t *>(ctl + kCtlHeader);
+    uint64_t *words = reinterpret_cast<uint64_t *>(ctl + kCtlHeader);
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    const Geo &g = in.geo;
+    const int64_t b = blockIdx.x;
+    const int64_t lo = b * g.nseg / nblocks, hi = (b + 1) * g.nseg / nblocks;
 
-    for (int i = tid; i < 256; i += kBlock) s_case[i] = ALGO == DISO_MC ? kMcCase[i] : kDmcCase[i];
+    for (int i = tid; i < 256; i += kCountBlock) s_case[i] = ALGO == DISO_MC ? kMcCase[i] : kDmcCase[i];
     if (tid == 0) {
-        s_bid = atomicAdd(&hdr->ticket, 1u);
         s_parity = __hip_atomic_load(&hdr->parity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_dirty_other =
             __hip_atomic_load(&hdr->dirty[s_parity ^ 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    const int64_t bid = s_bid;
     const uint32_t parity = s_parity;
-    const Geo &g = in.geo;
+    uint64_t *cur = words + (int64_t)parity * kMaxCountBlocks;
 
-    int64_t segv[kSegsPerWaveCount], segf[kSegsPerWaveCount];
+    // 1. Count: wave w takes segments lo+w, lo+w+8, ...; lane 0 records each segment's counts.
     int64_t wv = 0, wf = 0;
     uint32_t wfl = 0;
-#pragma unroll
-    for (int k = 0; k < kSegsPerWaveCount; ++k) {
-        int64_t s = bid * kSegsPerBlockCount + wave * kSegsPerWaveCount + k;
-        int nv = 0, nf = 0;
-        uint32_t fl = 0;
-        if (s < g.nseg) {  // wave-uniform
-            int chunk = (int)(s % g.nchunk);
-            int64_t row = s / g.nchunk;
-            int px = (int)(row / g.g1), py = (int)(row % g.g1);
-            count_position<S, ALGO>(in, s_case, px, py, chunk * kWave + lane, &nv, &nf, &fl);
-        }
+    for (int64_t s = lo + wave; s < hi; s += kCountWaves) {  // wave-uniform
+        const int chunk = (int)(s % g.nchunk);
+        const int64_t row = s / g.nchunk;
+        int nv, nf;
+        uint32_t fl;
+        count_position<S, ALGO>(in, s_case, (int)(row / g.g1), (int)(row % g.g1), chunk * kWave + lane, &nv,
+                                &nf, &fl);
         int tv, tf;
         (void)wave_excl_small(nv, &tv);
         (void)wave_excl_small(nf, &tf);
-        segv[k] = tv;
-        segf[k] = tf;
+        if (lane == 0) {
+            seg[s].v = tv;
+            seg[s].f = tf;
+        }
         wv += tv;
         wf += tf;
-        wfl |= wave_or(fl);
+        wfl |= fl;
     }
+    wfl = wave_or(wfl);
     if (lane == 0) {
-        s_wv[wave] = wv;
-        s_wf[wave] = wf;
-        s_wfl[wave] = wfl;
+        s_v[wave] = wv;
+        s_f[wave] = wf;
+        s_fl[wave] = wfl;
     }
     __syncthreads();
-
-    if (wave == 0) {
-        int64_t av = 0, af = 0;
-        uint32_t afl = 0;
+    int64_t av = 0, af = 0;
+    uint32_t afl = 0;
 #pragma unroll
-        for (int w = 0; w < kWavesPerBlock; ++w) {
-            av += s_wv[w];
-            af += s_wf[w];
-            afl |= s_wfl[w];
-        }
-        uint64_t *tv_word = tiles + ((int64_t)parity * cap + bid) * 2;
-        uint64_t *tf_word = tv_word + 1;
-        const uint64_t flbits = (uint64_t)afl << 59;
-        if (lane == 0) {
-            uint64_t st = bid == 0 ? kStP : kStA;
-            st_relaxed(tv_word, st | (uint64_t)av);
-            st_relaxed(tf_word, st | flbits | (uint64_t)af);
-        }
-        // Look-back: lane i inspects predecessor (pred - i); sum aggregates down to the nearest
-        // inclusive prefix, independently for the vertex and the face count.
-        int64_t ev = 0, ef = 0;
-        uint32_t efl = 0;
-        bool timeout = false;
-        if (bid > 0) {
-            bool done_v = false, done_f = false;
-            int64_t pred = bid - 1;
-            uint32_t spins = 0;
-            while (!(done_v && done_f)) {
-                int64_t j = pred - lane;
-                uint64_t wvj = kStP, wfj = kStP;
-                if (j >= 0) {
-                    const uint64_t *pj = tiles + ((int64_t)parity * cap + j) * 2;
-                    do {
-                        wvj = ld_relaxed(pj);
-                        wfj = ld_relaxed(pj + 1);
-                    } while ((st_of(wvj) == 0 || st_of(wfj) == 0) && ++spins < kSpinLimit);
-                }
-                if (__ballot(spins >= kSpinLimit)) {
-                    timeout = true;
-                    break;
-                }
-                if (!done_v) {
-                    uint64_t mp = __ballot(st_of(wvj) == 2);
-                    int first = mp ? __ffsll((unsigned long long)mp) - 1 : kWave;
-                    ev += wave_sum64(lane <= first ? val_of(wvj) : 0);
-                    done_v = mp != 0;
-                }
-                if (!done_f) {
-                    uint64_t mp = __ballot(st_of(wfj) == 2);
-                    int first = mp ? __ffsll((unsigned long long)mp) - 1 : kWave;
-                    ef += wave_sum64(lane <= first ? val_of(wfj) : 0);
-                    efl |= wave_or(lane <= first ? fl_of(wfj) : 0u);
-                    done_f = mp != 0;
-                }
-                pred -= kWave;
-            }
-        }
-        if (lane == 0) {
-            if (bid > 0) {
-                st_relaxed(tv_word, kStP | (uint64_t)(ev + av));
-                st_relaxed(tf_word, kStP | ((uint64_t)(efl | afl) << 59) | (uint64_t)(ef + af));
-            }
-            s_ev = ev;
-            s_ef = ef;
-            if (bid == nblocks - 1) {
-                SegBase *tot = seg + g.nseg;
-                tot[0].v = ev + av;
-                tot[0].f = ef + af;
-                tot[1].v = (int64_t)(efl | afl | (timeout ? DISO_TOTAL_SCAN_TIMEOUT : 0u));
-                tot[1].f = ALGO;
-                // Every block has taken its ticket and published its aggregate by now.
-                __hip_atomic_store(&hdr->dirty[parity], (uint32_t)nblocks, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&hdr->dirty[parity ^ 1u], 0u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&hdr->parity, parity ^ 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&hdr->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
+    for (int w = 0; w < kCountWaves; ++w) {
+        av += s_v[w];
+        af += s_f[w];
+        afl |= s_fl[w];
+    }
+    if (tid == 0) st_relaxed(cur + b, pack_agg(av, af, afl));
+
+    // 2. Exclusive prefix of this block = sum of all predecessors' aggregates.
+    int64_t pv = 0, pf = 0;
+    uint32_t pfl = 0;
+    bool timeout = false;
+    for (int64_t j = tid; j < b; j += kCountBlock) {
+        uint64_t w;
+        uint32_t spins = 0;
+        do {
+            w = ld_relaxed(cur + j);
+        } while ((w >> 62) == 0 && ++spins < kSpinLimit);
+        timeout |= (w >> 62) == 0;
+        pv += (int64_t)(w & ((1ull << 29) - 1));
+        pf += (int64_t)((w >> 29) & ((1ull << 30) - 1));
+        pfl |= (uint32_t)((w >> 59) & 7);
+    }
+    pv = wave_sum64(pv);
+    pf = wave_sum64(pf);
+    pfl = wave_or(pfl);
+    const bool wave_timeout = __ballot(timeout) != 0;
+    __syncthreads();  // s_v / s_f reuse
+    if (lane == 0) {
+        s_v[wave] = pv;
+        s_f[wave] = pf;
+        s_fl[wave] = pfl | (wave_timeout ? DISO_TOTAL_SCAN_TIMEOUT : 0u);
     }
     __syncthreads();
+    int64_t ev = 0, ef = 0;
+    uint32_t efl = 0;
+#pragma unroll
+    for (int w = 0; w < kCountWaves; ++w) {
+        ev += s_v[w];
+        ef += s_f[w];
+        efl |= s_fl[w];
+    }
+    if (b == nblocks - 1 && tid == 0) {
+        SegBase *tot = seg + g.nseg;
+        tot[0].v = ev + av;
+        tot[0].f = ef + af;
+        tot[1].v = (int64_t)(efl | afl);
+        tot[1].f = ALGO;
+        // Every block has read the parity (it published a word, which this block waited for).
+        __hip_atomic_store(&hdr->dirty[parity], (uint32_t)nblocks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&hdr->dirty[parity ^ 1u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&hdr->parity, parity ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 
-    // Clear this block's share of the other tile buffer for the next call.
+    // 3. Clear this block's share of the other word buffer for the next call.
     {
-        int64_t n = s_dirty_other;
-        int64_t per = (n + nblocks - 1) / nblocks;
-        int64_t lo = bid * per, hi = lo + per < n ? lo + per : n;
-        uint64_t *other = tiles + (int64_t)(parity ^ 1u) * cap * 2;
-        for (int64_t i = lo * 2 + tid; i < hi * 2; i += kBlock) st_relaxed(other + i, 0ull);
+        const int64_t n = s_dirty_other;
+        const int64_t per = (n + nblocks - 1) / nblocks;
+        const int64_t c0 = b * per, c1 = c0 + per < n ? c0 + per : n;
+        uint64_t *other = words + (int64_t)(parity ^ 1u) * kMaxCountBlocks;
+        for (int64_t i = c0 + tid; i < c1; i += kCountBlock) st_relaxed(other + i, 0ull);
     }
 
-    // Segment bases.
-    if (lane == 0) {
-        int64_t bv = s_ev, bf = s_ef;
-        for (int w = 0; w < wave; ++w) {
-            bv += s_wv[w];
-            bf += s_wf[w];
+    // 4. Segment bases: block-wide exclusive scan of this block's segment counts.
+    __syncthreads();  // s_v / s_f reuse; phase-1 counts complete
+    for (int64_t base = lo; base < hi; base += kCountBlock) {
+        const int64_t s = base + tid;
+        int64_t cv = 0, cf = 0;
+        if (s < hi) {
+            cv = (int64_t)__hip_atomic_load(reinterpret_cast<uint64_t *>(&seg[s].v), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+            cf = (int64_t)__hip_atomic_load(reinterpret_cast<uint64_t *>(&seg[s].f), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
         }
+        const int64_t iv = wave_incl64(cv), jf = wave_incl64(cf);
+        if (lane == kWave - 1) {
+            s_v[wave] = iv;
+            s_f[wave] = jf;
+        }
+        __syncthreads();
+        int64_t ov = ev, of = ef, tv = 0, tf = 0;
 #pragma unroll
-        for (int k = 0; k < kSegsPerWaveCount; ++k) {
-            int64_t s = bid * kSegsPerBlockCount + wave * kSegsPerWaveCount + k;
-            if (s < g.nseg) {
-                seg[s].v = bv;
-                seg[s].f = bf;
+        for (int w = 0; w < kCountWaves; ++w) {
+            if (w < wave) {
+                ov += s_v[w];
+                of += s_f[w];
             }
-            bv += segv[k];
-            bf += segf[k];
+            tv += s_v[w];
+            tf += s_f[w];
         }
+        if (s < hi) {
+            seg[s].v = ov + iv - cv;
+            seg[s].f = of + jf - cf;
+        }
+        ev += tv;
+        ef += tf;
+        __syncthreads();
     }
 }
 
@@ -992,13 +996,31 @@ bool make_geo(int64_t X, int64_t Y, int64_t Z, int flags, Geo *g) {
     g->off = pad / 2;
     g->nchunk = (g->g2 + kWave - 1) / kWave;
     g->nseg = (int64_t)g->g0 * g->g1 * g->nchunk;
-    // Linear sample indices and counts must stay well inside int64 / the 59-bit scan words.
-    return (int64_t)g->g0 * g->g1 * g->g2 < (1ll << 40);
+    // Per-block aggregates must fit the 29/30-bit fields of the scan words.
+    return (int64_t)g->g0 * g->g1 * g->g2 < (1ll << 36);
 }
 
-int64_t count_blocks(const Geo &g) { return (g.nseg + kSegsPerBlockCount - 1) / kSegsPerBlockCount; }
 int64_t seg_bytes_for(const Geo &g) { return (g.nseg + 2) * (int64_t)sizeof(SegBase); }
-int64_t ctl_bytes_for(const Geo &g) { return kCtlHeader + count_blocks(g) * 2 * 2 * (int64_t)sizeof(uint64_t); }
+int64_t ctl_bytes_for(const Geo &) { return kCtlHeader + 2 * kMaxCountBlocks * (int64_t)sizeof(uint64_t); }
+
+// Number of count blocks: about kCountBlocksPerCU per CU of the current device, so that every
+// block is resident at once, and never more than the segments or the word buffer allow.
+int64_t count_blocks(const Geo &g) {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    int64_t nb = (int64_t)cus[dev] * kCountBlocksPerCU;
+    const int64_t min_nb = (g.nseg + (1ll << 20) - 1) >> 20;  // <= 2^20 segments per block keeps the
+    if (nb < min_nb) nb = min_nb;                            // aggregate fields from overflowing
+    if (nb > kMaxCountBlocks) nb = kMaxCountBlocks;
+    if (nb > g.nseg) nb = g.nseg;
+    return nb < 1 ? 1 : nb;
+}
 
 template <typename S>
 In<S> make_in(const void *sdf, const void *deform, double iso, const Geo &g) {
@@ -1016,13 +1038,13 @@ int launch_count(int algo, const void *sdf, double iso, const Geo &g, void *seg,
                  int64_t ctl_bytes, hipStream_t st) {
     In<S> in = make_in<S>(sdf, nullptr, iso, g);
     const int64_t nb = count_blocks(g);
-    const int64_t cap = (ctl_bytes - kCtlHeader) / (2 * 2 * (int64_t)sizeof(uint64_t));
+    (void)ctl_bytes;
     if (algo == DISO_MC)
-        hipLaunchKernelGGL((count_kernel<S, DISO_MC>), dim3((unsigned)nb), dim3(kBlock), 0, st, in,
-                           (SegBase *)seg, (uint8_t *)ctl, cap, nb);
+        hipLaunchKernelGGL((count_kernel<S, DISO_MC>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, in,
+                           (SegBase *)seg, (uint8_t *)ctl, nb);
     else
-        hipLaunchKernelGGL((count_kernel<S, DISO_DMC>), dim3((unsigned)nb), dim3(kBlock), 0, st, in,
-                           (SegBase *)seg, (uint8_t *)ctl, cap, nb);
+        hipLaunchKernelGGL((count_kernel<S, DISO_DMC>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, in,
+                           (SegBase *)seg, (uint8_t *)ctl, nb);
     return check_hip(hipGetLastError(), "count kernel launch");
 }
 
