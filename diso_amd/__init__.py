@@ -48,13 +48,16 @@ class _Extract(torch.autograd.Function):
     def forward(ctx, grid, deform, ex, counted, flags):
         verts, faces = ex.emit(grid, deform, counted, flags)
         ctx.ex, ctx.counted, ctx.flags = ex, counted, flags
-        ctx.save_for_backward(grid, deform)
+        # DMC backward reads the quads; keep them through autograd, not through `counted`
+        counted.quads = None
+        ctx.save_for_backward(grid, deform, faces if ex.algo == _lib.DMC else None)
         ctx.mark_non_differentiable(faces)
         return verts, faces
 
     @staticmethod
     def backward(ctx, adj_verts, adj_faces):
-        grid, deform = ctx.saved_tensors
+        grid, deform, quads = ctx.saved_tensors
+        ctx.counted.quads = quads
         grad_grid = torch.empty_like(grid)
         grad_deform = None if deform is None else torch.empty_like(deform)
         with torch.cuda.device(grid.device):

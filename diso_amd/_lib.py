@@ -34,7 +34,7 @@ TOTAL_HAS_ABOVE = 1 << 1
 TOTAL_HAS_NAN = 1 << 2
 
 EXPORTS = [
-    "diso_seg_bytes", "diso_ctl_bytes", "diso_count", "diso_emit", "diso_backward",
+    "diso_seg_bytes", "diso_ctl_bytes", "diso_scratch_bytes", "diso_count", "diso_emit", "diso_backward",
     "diso_read_totals", "diso_last_error", "diso_version",
 ]
 
@@ -78,11 +78,14 @@ def lib():
         L.diso_count.restype = i32
         L.diso_count.argtypes = [i32, i32, vp, i64, i64, i64, dbl, i32, vp, i64, vp, i64,
                                  ctypes.POINTER(Totals), vp]
+        L.diso_scratch_bytes.restype = i64
+        L.diso_scratch_bytes.argtypes = [i32, i32, i64]
         L.diso_emit.restype = i32
-        L.diso_emit.argtypes = [i32, i32, vp, vp, i64, i64, i64, dbl, i32, vp, i64, vp, i64, vp, i64, vp]
+        L.diso_emit.argtypes = [i32, i32, vp, vp, i64, i64, i64, dbl, i32, vp, i64, vp, i64, vp, i64, vp,
+                                vp, i64, vp]
         L.diso_backward.restype = i32
         L.diso_backward.argtypes = [i32, i32, vp, vp, i64, i64, i64, dbl, i32, vp, i64, vp, i64, vp, vp,
-                                    vp]
+                                    i64, vp, vp, vp, i64, vp]
         L.diso_read_totals.restype = i32
         L.diso_read_totals.argtypes = [vp, i64, i64, i64, i64, i32, ctypes.POINTER(Totals), vp]
         L.diso_last_error.restype = ctypes.c_char_p

@@ -125,30 +125,6 @@ __device__ __forceinline__ S interp_t(S d0, S d1, S iso) {
     return (d1 != d0) ? clamp01<S>((iso - d0) / (d1 - d0)) : S(0.5);  // src/cumc.cu:356
 }
 
-// computeMcVert (src/cumc.cu:343-368): p0 + (p1 - p0) * t with p = integer position + deform.
-template <typename S, bool DEF>
-__device__ __forceinline__ V3<S> edge_vertex(const In<S> &in, int x, int y, int z, int axis) {
-    int x1 = x + (axis == 0), y1 = y + (axis == 1), z1 = z + (axis == 2);
-    S d0 = sample(in, x, y, z), d1 = sample(in, x1, y1, z1);
-    S t = interp_t(d0, d1, in.iso);
-    S p0x = S(x), p0y = S(y), p0z = S(z);
-    S p1x = S(x1), p1y = S(y1), p1z = S(z1);
-    if (DEF) {
-        V3<S> a = deform_at<S, DEF>(in, x, y, z), b = deform_at<S, DEF>(in, x1, y1, z1);
-        p0x += a.x;
-        p0y += a.y;
-        p0z += a.z;
-        p1x += b.x;
-        p1y += b.y;
-        p1z += b.z;
-    }
-    V3<S> r;
-    r.x = p0x + (p1x - p0x) * t;
-    r.y = p0y + (p1y - p0y) * t;
-    r.z = p0z + (p1z - p0z) * t;
-    return r;
-}
-
 // The eight corner samples of the cell anchored at (x,y,z): c[dx][dy][dz].
 template <typename S>
 struct Corners {
@@ -208,27 +184,6 @@ __device__ __forceinline__ int dmc_good_code(const In<S> &in, const uint64_t *ca
         }
     }
     return code;
-}
-
-// Resolved DMC code of the cell anchored at (x,y,z), or 0 (no patches) when it is not a cell.
-template <typename S>
-__device__ __forceinline__ int dmc_cell(const In<S> &in, const uint64_t *cases, int x, int y, int z) {
-    if (!is_anchor(in, x, y, z)) return 0;
-    int code = dmc_code(load_corners(in, x, y, z), in.iso);
-    if (code == 0 || code == 255) return code;
-    return dmc_good_code(in, cases, code, x, y, z);
-}
-
-// Crossing flags (bit a = edge along axis a) of the edges owned by the cell at (x,y,z).
-template <typename S>
-__device__ __forceinline__ int owned_crossings(const In<S> &in, int x, int y, int z) {
-    if (!is_anchor(in, x, y, z)) return 0;
-    S d0 = sample(in, x, y, z);
-    int m = 0;
-    m |= crosses(d0, sample(in, x + 1, y, z), in.iso) << 0;
-    m |= crosses(d0, sample(in, x, y + 1, z), in.iso) << 1;
-    m |= crosses(d0, sample(in, x, y, z + 1), in.iso) << 2;
-    return m;
 }
 
 __device__ __forceinline__ int popc3(int m) { return (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1); }
@@ -507,7 +462,121 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *s
 }
 
 // ------------------------------------------------------------------------------------------
+// Register windows and edge geometry shared by the emission and backward kernels
+
+// Samples of a lane's neighbourhood held in registers: v[i][j][k] is the sample at
+// (px + X0 + i, py + Y0 + j, pz + Z0 + k). Only ever indexed with compile-time constants.
+template <typename S, int X0, int NX, int Y0, int NY, int Z0, int NZ>
+struct Win {
+    S v[NX][NY][NZ];
+    __device__ __forceinline__ void load(const In<S> &in, int px, int py, int pz) {
+        bool okx[NX], oky[NY], okz[NZ];
+#pragma unroll
+        for (int i = 0; i < NX; ++i) okx[i] = (unsigned)(px + X0 + i - in.geo.off) < (unsigned)in.geo.n0;
+#pragma unroll
+        for (int j = 0; j < NY; ++j) oky[j] = (unsigned)(py + Y0 + j - in.geo.off) < (unsigned)in.geo.n1;
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) okz[k] = (unsigned)(pz + Z0 + k - in.geo.off) < (unsigned)in.geo.n2;
+        const int64_t sy = in.geo.n2, sx = (int64_t)in.geo.n1 * in.geo.n2;
+        const int64_t base = (int64_t)(px + X0 - in.geo.off) * sx + (int64_t)(py + Y0 - in.geo.off) * sy +
+                             (pz + Z0 - in.geo.off);
+#pragma unroll
+        for (int i = 0; i < NX; ++i)
+#pragma unroll
+            for (int j = 0; j < NY; ++j)
+#pragma unroll
+                for (int k = 0; k < NZ; ++k)
+                    v[i][j][k] = (okx[i] && oky[j] && okz[k]) ? in.sdf[base + i * sx + j * sy + k] : in.pad;
+    }
+    __device__ __forceinline__ S operator()(int dx, int dy, int dz) const { return v[dx - X0][dy - Y0][dz - Z0]; }
+};
+
+// Edge e of a cell (src/cumc.cu:109-122): starts at cell-relative corner (eox, eoy, eoz) and
+// runs along axis eax. Bit masks of the edge table, so unrolled loops fold them to constants.
+__device__ constexpr int eox(int e) { return (0x622 >> e) & 1; }
+__device__ constexpr int eoy(int e) { return (0x0F0 >> e) & 1; }
+__device__ constexpr int eoz(int e) { return (0xC44 >> e) & 1; }
+__device__ constexpr int eax(int e) { return e >= 8 ? 1 : ((e & 1) ? 2 : 0); }
+
+// Crossing mask of the three edges a cell owns, from its sample and its +x, +y, +z neighbours.
+template <typename S>
+__device__ __forceinline__ int own_mask(S d0, S dx, S dy, S dz, S iso) {
+    return crosses(d0, dx, iso) | (crosses(d0, dy, iso) << 1) | (crosses(d0, dz, iso) << 2);
+}
+
+// DMC code of the cell anchored at window offset (cx,cy,cz) (corners cx..cx+1 etc.), resolved
+// with the face-neighbour rule; 0 when the position is not a cell.
+template <typename S, typename W>
+__device__ __forceinline__ int dmc_code_w(const In<S> &in, const uint64_t *cases, const W &w, int px, int py,
+                                          int pz, int cx, int cy, int cz) {
+    if (!is_anchor(in, px + cx, py + cy, pz + cz)) return 0;
+    int code = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) code |= (w(cx + (i & 1), cy + ((i >> 1) & 1), cz + ((i >> 2) & 1)) >= in.iso) << i;
+    if (code == 0 || code == 255) return code;
+    return dmc_good_code(in, cases, code, px + cx, py + cy, pz + cz);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename S>
+__device__ __forceinline__ V3<S> add3(V3<S> a, V3<S> b) {
+    return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+
+// computeMcVert from samples and deforms already in registers: p0 + (p1 - p0) * t with
+// p = integer position + deform (src/cumc.cu:343-368).
+template <typename S, bool DEF>
+__device__ __forceinline__ V3<S> lerp_vertex(int x, int y, int z, int axis, S d0, S d1, V3<S> f0, V3<S> f1,
+                                             S iso) {
+    const S t = interp_t(d0, d1, iso);
+    S p0x = S(x), p0y = S(y), p0z = S(z);
+    S p1x = S(x + (axis == 0)), p1y = S(y + (axis == 1)), p1z = S(z + (axis == 2));
+    if (DEF) {
+        p0x += f0.x;
+        p0y += f0.y;
+        p0z += f0.z;
+        p1x += f1.x;
+        p1y += f1.y;
+        p1z += f1.z;
+    }
+    V3<S> r;
+    r.x = p0x + (p1x - p0x) * t;
+    r.y = p0y + (p1y - p0y) * t;
+    r.z = p0z + (p1z - p0z) * t;
+    return r;
+}
+
+// x / c for a divisor c fixed per call, with r = RN(1/c): q = x*r is within an ulp of x/c and
+// one FMA correction with the exact residual makes it the correctly rounded quotient
+// (Markstein's theorem), i.e. bit-identical to IEEE x / c, in 3 instructions instead of the
+// ~10 of a general correctly rounded division.
+template <typename S>
+__device__ __forceinline__ S div_by(S x, S c, S r) {
+    if (c == S(0)) return x / c;  // uniform; keeps IEEE inf/nan for a length-1 axis
+    const S q = x * r;
+    return fma(fma(-q, c, x), r, q);
+}
+
+// Per-axis divisors (dims - 1) and their reciprocals, for the wrapper's normalisation.
+template <typename S>
+struct Norm {
+    S c[3], r[3];
+};
+
+// ------------------------------------------------------------------------------------------
+// ------------------------------------------------------------------------------------------
 // Pass 2: emission
+//
+// Work compaction: each lane publishes its work items (MC vertices and triangle corners; DMC
+// patch vertices and quads) into a dense per-wave list in LDS at its in-wave prefix, and the wave
+// then processes the list 64 items at a time. Every item is computed exactly once (a lane-per-
+// cell loop would make the whole wave execute the union of all lanes' branches), and item k is
+// the segment's k-th output row, so consecutive lanes write consecutive rows (coalesced stores).
 
 struct OutCfg {
     int flags;
@@ -515,44 +584,100 @@ struct OutCfg {
 };
 
 template <typename S>
-__device__ __forceinline__ void store_vertex(S *verts, int64_t i, V3<S> p, int flags, S nx, S ny,
-                                             S nz) {
+__device__ __forceinline__ V3<S> epilogue(V3<S> p, int flags, const Norm<S> &nm) {
     if (flags & DISO_FLAG_SHIFT) {  // verts - 1 (diso/__init__.py:56)
         p.x = p.x - S(1);
         p.y = p.y - S(1);
         p.z = p.z - S(1);
     }
     if (flags & DISO_FLAG_NORMALIZE) {  // verts / (dims - 1) (diso/__init__.py:57-60)
-        p.x = p.x / nx;
-        p.y = p.y / ny;
-        p.z = p.z / nz;
+        p.x = div_by(p.x, nm.c[0], nm.r[0]);
+        p.y = div_by(p.y, nm.c[1], nm.r[1]);
+        p.z = div_by(p.z, nm.c[2], nm.r[2]);
     }
-    verts[3 * i + 0] = p.x;
-    verts[3 * i + 1] = p.y;
-    verts[3 * i + 2] = p.z;
+    return p;
 }
 
-__device__ __forceinline__ void store_index(void *faces, int64_t i, int64_t v, int flags) {
+__device__ __forceinline__ void store_index(void *__restrict__ faces, int64_t i, int64_t v, int flags) {
     if (flags & DISO_FLAG_INDEX64)
         reinterpret_cast<int64_t *>(faces)[i] = v;
     else
         reinterpret_cast<int32_t *>(faces)[i] = (int32_t)v;
 }
 
-// MC: vertices of the owned crossing edges in axis order, then triangles. A triangle corner on
-// table edge e lies on an edge owned by the cell at (cell + DISO_EDGE_OFS[e]); its vertex id is
-// that owner's first vertex plus the rank of the edge's axis among the owner's crossing axes
-// (what create_cell_mc_tris_kernel finds by linear search, src/cumc.cu:586-608). Owners are
-// the cell itself, its +x, +y, +x+y neighbours (rows R00, R10, R01, R11 of this chunk) and their
-// +z neighbours, whose first vertex is the current one's first vertex plus its count.
-template <typename S, bool DEF>
-__global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase *seg, S *verts,
-                                                         void *faces, OutCfg out, S nx, S ny, S nz,
-                                                         int64_t nblocks) {
+__device__ __forceinline__ int64_t load_index(const void *__restrict__ faces, int64_t i, int flags) {
+    return (flags & DISO_FLAG_INDEX64) ? reinterpret_cast<const int64_t *>(faces)[i]
+                                       : (int64_t) reinterpret_cast<const int32_t *>(faces)[i];
+}
+
+template <typename S>
+__device__ __forceinline__ void store_vertex(S *__restrict__ verts, int64_t i, int64_t cap, V3<S> p) {
+    if (i < cap) {
+        verts[3 * i + 0] = p.x;
+        verts[3 * i + 1] = p.y;
+        verts[3 * i + 2] = p.z;
+    }
+}
+
+// The vertex ids of a cell's 12 edges, relative to its segment's first MC vertex. A corner on
+// table edge e lies on an edge owned by the cell at (cell + edge offset); its id is that owner's
+// first vertex plus the rank of the edge's axis among the owner's crossing axes (what
+// create_cell_mc_tris_kernel finds by linear search, src/cumc.cu:586-608). Owners are the cell,
+// its +x, +y, +x+y neighbours (rows R00, R10, R01, R11 of the chunk; rNN = first vertex, nNN =
+// count, mNN = crossing mask) and their +z neighbours, whose first vertex is the current one's
+// first vertex plus its count (cNNn = x-crossing flag of the +z neighbour).
+__device__ __forceinline__ void edge_ids(int32_t *c, int32_t r00, int32_t r10, int32_t r01, int32_t r11, int m00,
+                                         int m10, int m01, int m11, int c00n, int c10n) {
+    const int n00 = popc3(m00), n10 = popc3(m10), n01 = popc3(m01), n11 = popc3(m11);
+    c[0] = r00;
+    c[1] = r10 + n10 - 1;
+    c[2] = r00 + n00;
+    c[3] = r00 + n00 - 1;
+    c[4] = r01;
+    c[5] = r11 + n11 - 1;
+    c[6] = r01 + n01;
+    c[7] = r01 + n01 - 1;
+    c[8] = r00 + (m00 & 1);
+    c[9] = r10 + (m10 & 1);
+    c[10] = r10 + n10 + c10n;
+    c[11] = r00 + n00 + c00n;
+}
+
+// Crossing masks of the rows R10, R01, R11 and +z flags for edge_ids, from a window that covers
+// offsets x, y in [0, 2] and z in [0, 1].
+template <typename S, typename W>
+__device__ __forceinline__ void neighbour_masks(const In<S> &in, const W &w, int px, int py, int pz, int *m10,
+                                                int *m01, int *m11, int *c00n, int *c10n) {
+    const S iso = in.iso;
+    *m10 = is_anchor(in, px + 1, py, pz) ? own_mask(w(1, 0, 0), w(2, 0, 0), w(1, 1, 0), w(1, 0, 1), iso) : 0;
+    *m01 = is_anchor(in, px, py + 1, pz) ? own_mask(w(0, 1, 0), w(1, 1, 0), w(0, 2, 0), w(0, 1, 1), iso) : 0;
+    *m11 = is_anchor(in, px + 1, py + 1, pz) ? own_mask(w(1, 1, 0), w(2, 1, 0), w(1, 2, 0), w(1, 1, 1), iso) : 0;
+    *c00n = is_anchor(in, px, py, pz + 1) ? crosses(w(0, 0, 1), w(1, 0, 1), iso) : 0;
+    *c10n = is_anchor(in, px + 1, py, pz + 1) ? crosses(w(1, 0, 1), w(2, 0, 1), iso) : 0;
+}
+
+// MC emission (TRI) or MC edge vertices only (!TRI, for DMC). Vertex item = (lane, axis) of an
+// owned crossing edge, in the reference's order (cell, then axis x < y < z;
+// src/cumc.cu:370-410); triangle corner item = (lane, table edge).
+template <typename S>
+struct McEmitLds {
+    S d[4][kWave];            // sample, +x, +y, +z sample of each lane's cell
+    int32_t cand[kWave][12];  // vertex id of each table edge, relative to the segment's first vertex
+    uint16_t corner[kWave * 15];
+    uint8_t vert[kWave * 3];
+};
+
+template <typename S, bool DEF, bool TRI>
+__global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase *__restrict__ seg,
+                                                         S *__restrict__ verts, void *__restrict__ faces,
+                                                         OutCfg out, Norm<S> nm, int64_t nblocks) {
     __shared__ uint64_t s_case[256];
+    __shared__ McEmitLds<S> s_lds[kWavesPerBlock];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
-    for (int i = tid; i < 256; i += kBlock) s_case[i] = kMcCase[i];
-    __syncthreads();
+    if (TRI) {
+        for (int i = tid; i < 256; i += kBlock) s_case[i] = kMcCase[i];
+        __syncthreads();
+    }
     const Geo &g = in.geo;
     const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
     if (s >= g.nseg) return;
@@ -560,110 +685,181 @@ __global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase
     const int64_t row = s / g.nchunk;
     const int px = (int)(row / g.g1), py = (int)(row % g.g1);
     if (px >= g.g0 - 1 || py >= g.g1 - 1) return;  // no cells on this row (wave-uniform)
-    const int pz = chunk * kWave + lane;
-    const SegBase *tot = seg + g.nseg;
-    const bool fits = tot[0].v <= out.vcap && tot[0].f <= out.fcap;
+    const int z0 = chunk * kWave, pz = z0 + lane;
+    const int64_t vbase = seg[s].v, tbase = seg[s].f;
 
-    const int m00 = owned_crossings(in, px, py, pz);
-    const int m10 = owned_crossings(in, px + 1, py, pz);
-    const int m01 = owned_crossings(in, px, py + 1, pz);
-    const int m11 = owned_crossings(in, px + 1, py + 1, pz);
-    int t;
-    const int64_t vb00 = seg[s].v + wave_excl_small(popc3(m00), &t);
-    const int64_t vb10 = seg[seg_index(g, px + 1, py, chunk)].v + wave_excl_small(popc3(m10), &t);
-    const int64_t vb01 = seg[seg_index(g, px, py + 1, chunk)].v + wave_excl_small(popc3(m01), &t);
-    const int64_t vb11 = seg[seg_index(g, px + 1, py + 1, chunk)].v + wave_excl_small(popc3(m11), &t);
-    const int n00 = popc3(m00), n10 = popc3(m10), n01 = popc3(m01), n11 = popc3(m11);
-
-    int ntri = 0;
-    uint64_t w = 0;
-    if (is_anchor(in, px, py, pz)) {
-        w = s_case[mc_code(load_corners(in, px, py, pz), in.iso)];
-        ntri = DISO_MC_NTRI(w);
+    Win<S, 0, TRI ? 3 : 2, 0, TRI ? 3 : 2, 0, 2> w;
+    w.load(in, px, py, pz);
+    const S iso = in.iso;
+    const bool anchor = is_anchor(in, px, py, pz);
+    const int m00 = anchor ? own_mask(w(0, 0, 0), w(1, 0, 0), w(0, 1, 0), w(0, 0, 1), iso) : 0;
+    uint64_t cw = 0;
+    if (TRI && anchor) {  // Bourke corner order (src/cumc.cu:107)
+        int code = 0;
+        code |= (w(0, 0, 0) >= iso) << 0;
+        code |= (w(1, 0, 0) >= iso) << 1;
+        code |= (w(1, 1, 0) >= iso) << 2;
+        code |= (w(0, 1, 0) >= iso) << 3;
+        code |= (w(0, 0, 1) >= iso) << 4;
+        code |= (w(1, 0, 1) >= iso) << 5;
+        code |= (w(1, 1, 1) >= iso) << 6;
+        code |= (w(0, 1, 1) >= iso) << 7;
+        cw = s_case[code];
     }
-    const int64_t tb = seg[s].f + wave_excl_small(ntri, &t);
-    if (!fits) return;
+    const int ntri = DISO_MC_NTRI(cw);
+    // Most segments lie away from the surface: leave before any ballot.
+    if (__ballot(m00 != 0 || ntri != 0) == 0) return;
+    int tv, tt = 0;
+    const int p00 = wave_excl_small(popc3(m00), &tv);
+    const SegBase *tot = seg + g.nseg;
+    if (tot[0].v > out.vcap || (TRI && tot[0].f > out.fcap)) return;
 
-    // vertices
+    McEmitLds<S> &L = s_lds[wave];
+    L.d[0][lane] = w(0, 0, 0);
+    L.d[1][lane] = w(1, 0, 0);
+    L.d[2][lane] = w(0, 1, 0);
+    L.d[3][lane] = w(0, 0, 1);
     {
-        int64_t v = vb00;
+        int r = p00;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if ((m00 >> a) & 1) {
-                if (v < out.vcap) store_vertex(verts, v, edge_vertex<S, DEF>(in, px, py, pz, a), out.flags, nx, ny, nz);
-                ++v;
-            }
+        for (int a = 0; a < 3; ++a)
+            if ((m00 >> a) & 1) L.vert[r++] = (uint8_t)(lane * 4 + a);
+    }
+    if (TRI) {
+        int m10, m01, m11, c00n, c10n, t;
+        neighbour_masks(in, w, px, py, pz, &m10, &m01, &m11, &c00n, &c10n);
+        const int pt = wave_excl_small(ntri, &tt);
+        const int32_t r10 = (int32_t)(seg[seg_index(g, px + 1, py, chunk)].v - vbase) + wave_excl_small(popc3(m10), &t);
+        const int32_t r01 = (int32_t)(seg[seg_index(g, px, py + 1, chunk)].v - vbase) + wave_excl_small(popc3(m01), &t);
+        const int32_t r11 = (int32_t)(seg[seg_index(g, px + 1, py + 1, chunk)].v - vbase) + wave_excl_small(popc3(m11), &t);
+        if (ntri) {
+            edge_ids(L.cand[lane], p00, r10, r01, r11, m00, m10, m01, m11, c00n, c10n);
+            for (int i = 0; i < 3 * ntri; ++i) L.corner[3 * pt + i] = (uint16_t)((lane << 4) | DISO_MC_EDGE(cw, i));
         }
     }
-    // triangles
-    if (ntri) {
-        const int c00n = owned_crossings(in, px, py, pz + 1) & 1;
-        const int c10n = owned_crossings(in, px + 1, py, pz + 1) & 1;
-        for (int i = 0; i < 3 * ntri; ++i) {
-            int64_t id;
-            switch (DISO_MC_EDGE(w, i)) {
-                case 0: id = vb00; break;
-                case 1: id = vb10 + n10 - 1; break;
-                case 2: id = vb00 + n00; break;
-                case 3: id = vb00 + n00 - 1; break;
-                case 4: id = vb01; break;
-                case 5: id = vb11 + n11 - 1; break;
-                case 6: id = vb01 + n01; break;
-                case 7: id = vb01 + n01 - 1; break;
-                case 8: id = vb00 + (m00 & 1); break;
-                case 9: id = vb10 + (m10 & 1); break;
-                case 10: id = vb10 + n10 + c10n; break;
-                default: id = vb00 + n00 + c00n; break;
-            }
-            int64_t o = 3 * tb + i;
-            if (o < 3 * out.fcap) store_index(faces, o, id, out.flags);
+    wave_lds_sync();
+
+    for (int k = lane; k < tv; k += kWave) {
+        const int it = L.vert[k], src = it >> 2, a = it & 3;
+        const int z = z0 + src;
+        const V3<S> f0 = deform_at<S, DEF>(in, px, py, z);
+        const V3<S> f1 = deform_at<S, DEF>(in, px + (a == 0), py + (a == 1), z + (a == 2));
+        const V3<S> p = lerp_vertex<S, DEF>(px, py, z, a, L.d[0][src], L.d[1 + a][src], f0, f1, iso);
+        store_vertex(verts, vbase + k, out.vcap, epilogue(p, out.flags, nm));
+    }
+    if (TRI) {
+        const int64_t f0 = 3 * tbase, flim = 3 * out.fcap;
+        for (int k = lane; k < 3 * tt; k += kWave) {
+            const int it = L.corner[k];
+            if (f0 + k < flim) store_index(faces, f0 + k, vbase + L.cand[it >> 4][it & 15], out.flags);
         }
     }
 }
 
-// DMC: patch vertices of the cell (mean of the patch's edge vertices, src/cudualmc.cu:907-955)
-// and one quad per owned crossing edge (src/cudualmc.cu:1027-1056). A quad corner is the vertex
-// of the patch that holds the edge in one of the four cells around it; those cells are the
-// cell itself and its -x, -y, -z neighbours (rows R00, Rm0, R0m, Rmm at this z and z-1).
+// DMC emission from the MC edge vertices `mcv` (written by emit_mc_kernel<.., false> in MC
+// vertex order). Patch-vertex item = (lane, patch): the mean of the patch's MC edge vertices,
+// summed from zero in ascending edge order and scaled by 1/num (src/cudualmc.cu:907-955) - the
+// same values the reference recomputes per cell, read once from mcv. Quad item = (lane, axis,
+// leaving) of an owned crossing edge (src/cudualmc.cu:1027-1056): its corners are the vertices
+// of the patches holding the edge in the four cells around it, which lie in rows R00, Rm0 (-x),
+// R0m (-y), Rmm (-x-y) at this z and z-1.
 struct DmcCell {
     int code;      // resolved code (0 when not a cell)
     int64_t base;  // first DMC vertex of the cell
 };
 
-template <typename S>
-__device__ __forceinline__ int64_t dmc_row_base(const SegBase *seg, const Geo &g, int px, int py,
-                                                int chunk) {
+__device__ __forceinline__ int64_t row_patch_base(const SegBase *__restrict__ seg, const Geo &g, int px, int py,
+                                                  int chunk) {
     return (px >= 0 && py >= 0) ? seg[seg_index(g, px, py, chunk)].f : 0;
 }
 
-// Resolved codes and first-vertex ids of row (px,py) at this lane's z and at z-1.
+// The cells (cx,cy,cz) in {-1,0}^3 around a lane's position, indexed ci = 4*(-cx) + 2*(-cy) + (-cz)
+// (cell 7 is not computed): codes from the window, first-vertex ids from the segment bases and
+// the in-wave prefix of patch counts.
+struct DmcCells {
+    DmcCell c[8];
+};
+
+template <typename S, typename W>
+__device__ __forceinline__ DmcCells dmc_cells(const In<S> &in, const uint64_t *cases, const SegBase *__restrict__ seg,
+                                              const W &w, int px, int py, int pz, int chunk) {
+    DmcCells r;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int cur = dmc_code_w(in, cases, w, px, py, pz, -i, -j, 0);
+            int t;
+            const int64_t b =
+                row_patch_base(seg, in.geo, px - i, py - j, chunk) + wave_excl_small(DISO_DMC_NPATCH(cases[cur]), &t);
+            r.c[4 * i + 2 * j].code = cur;
+            r.c[4 * i + 2 * j].base = b;
+            const int prev = (i && j) ? 0 : dmc_code_w(in, cases, w, px, py, pz, -i, -j, -1);
+            r.c[4 * i + 2 * j + 1].code = prev;
+            r.c[4 * i + 2 * j + 1].base = b - DISO_DMC_NPATCH(cases[prev]);
+        }
+    return r;
+}
+
+// Quad stencil (src/cudualmc.cu:504-513) as cell index ci and edge id: entry k of type ty.
+// Types 3..5 (edges leaving the inside) list the cells of types 0..2 in the order 0,3,2,1.
+__device__ constexpr int stencil_ci(int ty, int k) {
+    constexpr int q[3][4] = {{0, 2, 3, 1}, {0, 1, 5, 4}, {0, 4, 6, 2}};
+    return ty < 3 ? q[ty][k] : q[ty - 3][(4 - k) & 3];
+}
+__device__ constexpr int stencil_eid(int ty, int k) {
+    constexpr int q[3][4] = {{0, 4, 6, 2}, {8, 11, 10, 9}, {3, 1, 5, 7}};
+    return ty < 3 ? q[ty][k] : q[ty - 3][(4 - k) & 3];
+}
+
 template <typename S>
-__device__ __forceinline__ void dmc_row(const In<S> &in, const uint64_t *cases, const SegBase *seg,
-                                        int px, int py, int chunk, int pz, DmcCell *cur,
-                                        DmcCell *prev) {
-    const int code = dmc_cell(in, cases, px, py, pz);
-    const int np = DISO_DMC_NPATCH(cases[code]);
-    int t;
-    cur->code = code;
-    cur->base = dmc_row_base<S>(seg, in.geo, px, py, chunk) + wave_excl_small(np, &t);
-    int pcode = __shfl_up(code, 1, kWave);
-    if (lane_id() == 0) pcode = dmc_cell(in, cases, px, py, pz - 1);
-    prev->code = pcode;
-    prev->base = cur->base - DISO_DMC_NPATCH(cases[pcode]);
+struct DmcEmitLds {
+    int32_t cand[kWave][12];   // MC vertex id of each of the cell's edges, relative to the segment
+    uint8_t code[8][kWave];    // resolved codes of the cells around the lane
+    int32_t base[8][kWave];    // their first DMC vertex, relative to the segment's first
+    uint8_t patch[kWave * 4];  // patch items: lane * 4 + patch
+    uint16_t quad[kWave * 3];  // quad items: lane * 8 + axis * 2 + leaving
+};
+
+// Block-wide DMC tables: case words, edge mask of each (case, patch), the quad stencil and the
+// reciprocals 1/n of the possible patch sizes.
+template <typename S>
+struct DmcTables {
+    uint64_t cases[256];
+    uint16_t pedges[256][4];
+    S rcp[8];
+    uint8_t st_ci[6][4], st_eid[6][4];
+};
+
+template <typename S>
+__device__ __forceinline__ void load_dmc_tables(DmcTables<S> &T, int tid) {
+    for (int i = tid; i < 256; i += kBlock) {
+        const uint64_t w = kDmcCase[i];
+        T.cases[i] = w;
+        for (int k = 0; k < 4; ++k) {
+            uint16_t m = 0;
+            for (int e = 0; e < 12; ++e) m |= (uint16_t)((DISO_DMC_PATCH_OF(w, e) == k) << e);
+            T.pedges[i][k] = m;
+        }
+    }
+    if (tid < 8) T.rcp[tid] = tid ? S(1) / S(tid) : S(0);
+    if (tid < 24) {
+        const int ty = tid / 4, k = tid % 4;
+        T.st_ci[ty][k] = (uint8_t)stencil_ci(ty, k);
+        T.st_eid[ty][k] = (uint8_t)stencil_eid(ty, k);
+    }
 }
 
-__device__ __forceinline__ int64_t dmc_vertex_id(const uint64_t *cases, const DmcCell &c, int eid) {
-    int p = DISO_DMC_PATCH_OF(cases[c.code], eid);
-    return c.base + (p == 7 ? -1 : p);  // dmcEdgeOffset -1 for a non-crossing edge
-}
-
-template <typename S, bool DEF>
-__global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBase *seg, S *verts,
-                                                          void *faces, OutCfg out, S nx, S ny, S nz,
-                                                          int64_t nblocks) {
-    __shared__ uint64_t s_case[256];
+template <typename S>
+__global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBase *__restrict__ seg,
+                                                          const S *__restrict__ mcv, int64_t mcv_cap,
+                                                          S *__restrict__ verts,
+                                                          void *__restrict__ faces, uint8_t *__restrict__ vinfo,
+                                                          OutCfg out, Norm<S> nm, int64_t nblocks) {
+    __shared__ DmcTables<S> T;
+    __shared__ DmcEmitLds<S> s_lds[kWavesPerBlock];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
-    for (int i = tid; i < 256; i += kBlock) s_case[i] = kDmcCase[i];
+    load_dmc_tables(T, tid);
     __syncthreads();
     const Geo &g = in.geo;
     const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
@@ -672,130 +868,120 @@ __global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBas
     const int64_t row = s / g.nchunk;
     const int px = (int)(row / g.g1), py = (int)(row % g.g1);
     if (px >= g.g0 - 1 || py >= g.g1 - 1) return;
-    const int pz = chunk * kWave + lane;
-    const SegBase *tot = seg + g.nseg;
-    const bool fits = tot[0].v <= out.fcap && tot[0].f <= out.vcap;
+    const int z0 = chunk * kWave, pz = z0 + lane;
+    const S iso = in.iso;
+    const int64_t qbase = seg[s].v, pbase = seg[s].f;
 
-    DmcCell c00, c00m, c0m, c0mm, cm0, cm0m, cmm, cmmm;
-    dmc_row(in, s_case, seg, px, py, chunk, pz, &c00, &c00m);
-    dmc_row(in, s_case, seg, px, py - 1, chunk, pz, &c0m, &c0mm);
-    dmc_row(in, s_case, seg, px - 1, py, chunk, pz, &cm0, &cm0m);
-    dmc_row(in, s_case, seg, px - 1, py - 1, chunk, pz, &cmm, &cmmm);
-    (void)cmmm;
-    const int m00 = owned_crossings(in, px, py, pz);
-    int t;
-    const int64_t qb = seg[s].v + wave_excl_small(popc3(m00), &t);
-    if (!fits) return;
-
-    // patch vertices
-    const uint64_t w = s_case[c00.code];
-    const int np = DISO_DMC_NPATCH(w);
-    if (np) {
-        V3<S> ev[12];
+    Win<S, -1, 4, -1, 4, -1, 3> w;  // x, y in [-1, 2], z in [-1, 1]
+    w.load(in, px, py, pz);
+    const bool anchor = is_anchor(in, px, py, pz);
+    const int m00 = anchor ? own_mask(w(0, 0, 0), w(1, 0, 0), w(0, 1, 0), w(0, 0, 1), iso) : 0;
+    {
+        // A cell with patches has mixed corners; leave early when no lane owns a crossing edge
+        // or a patch.
+        bool lo = false, hi = false;
 #pragma unroll
-        for (int e = 0; e < 12; ++e) {
-            if (DISO_DMC_PATCH_OF(w, e) != 7)
-                ev[e] = edge_vertex<S, DEF>(in, px + kEdgeOfs[e][0], py + kEdgeOfs[e][1],
-                                           pz + kEdgeOfs[e][2], kEdgeAxis[e]);
+        for (int i = 0; i < 8; ++i) {
+            const bool b = w(i & 1, (i >> 1) & 1, (i >> 2) & 1) >= iso;
+            lo |= !b;
+            hi |= b;
         }
-        for (int k = 0; k < np; ++k) {
-            S sx = S(0), sy = S(0), sz = S(0), num = S(0);
-#pragma unroll
-            for (int e = 0; e < 12; ++e) {
-                if (DISO_DMC_PATCH_OF(w, e) == k) {
-                    sx += ev[e].x;
-                    sy += ev[e].y;
-                    sz += ev[e].z;
-                    num += S(1);
-                }
-            }
-            S r = S(1) / num;
-            V3<S> p = {sx * r, sy * r, sz * r};
-            int64_t v = c00.base + k;
-            if (v < out.vcap) store_vertex(verts, v, p, out.flags, nx, ny, nz);
-        }
+        if (__ballot(m00 != 0 || (anchor && lo && hi)) == 0) return;
     }
-    // quads
-    if (m00) {
-        const S d0 = sample(in, px, py, pz);
-        int64_t q = qb;
+    const DmcCells cells = dmc_cells(in, T.cases, seg, w, px, py, pz, chunk);
+    const uint64_t cw = T.cases[cells.c[0].code];
+    const int np = DISO_DMC_NPATCH(cw);
+    int tq, tp, t;
+    const int pq = wave_excl_small(popc3(m00), &tq);
+    const int pp = wave_excl_small(np, &tp);
+    int m10, m01, m11, c00n, c10n;
+    neighbour_masks(in, w, px, py, pz, &m10, &m01, &m11, &c00n, &c10n);
+    const int32_t r10 = (int32_t)(seg[seg_index(g, px + 1, py, chunk)].v - qbase) + wave_excl_small(popc3(m10), &t);
+    const int32_t r01 = (int32_t)(seg[seg_index(g, px, py + 1, chunk)].v - qbase) + wave_excl_small(popc3(m01), &t);
+    const int32_t r11 = (int32_t)(seg[seg_index(g, px + 1, py + 1, chunk)].v - qbase) + wave_excl_small(popc3(m11), &t);
+    const SegBase *tot = seg + g.nseg;
+    if (tot[0].v > out.fcap || tot[0].f > out.vcap || tot[0].v > mcv_cap) return;
+
+    DmcEmitLds<S> &L = s_lds[wave];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        L.code[i][lane] = (uint8_t)cells.c[i].code;
+        L.base[i][lane] = (int32_t)(cells.c[i].base - pbase);
+    }
+    if (np) {
+        edge_ids(L.cand[lane], pq, r10, r01, r11, m00, m10, m01, m11, c00n, c10n);
+        for (int k = 0; k < np; ++k) L.patch[pp + k] = (uint8_t)(lane * 4 + k);
+    }
+    {
+        const S d0 = w(0, 0, 0);
+        int q = pq;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             if (!((m00 >> a) & 1)) continue;
-            const S d1 = sample(in, px + (a == 0), py + (a == 1), pz + (a == 2));
-            const bool exiting = d1 < in.iso && d0 >= in.iso;  // type a + 3 (src/cudualmc.cu:782-788)
-            int64_t id[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                int64_t v[2];
-#pragma unroll
-                for (int o = 0; o < 2; ++o) {
-                    const int *q4 = kQuad[a + 3 * o][k];
-                    const int sel = (-q4[0]) * 4 + (-q4[1]) * 2 + (-q4[2]);
-                    const DmcCell &c = sel == 0 ? c00 : sel == 1 ? c00m : sel == 2 ? c0m : sel == 3 ? c0mm
-                                     : sel == 4 ? cm0 : sel == 5 ? cm0m : cmm;
-                    v[o] = dmc_vertex_id(s_case, c, q4[3]);
-                }
-                id[k] = exiting ? v[1] : v[0];
+            const S d1 = a == 0 ? w(1, 0, 0) : (a == 1 ? w(0, 1, 0) : w(0, 0, 1));
+            const int leaving = d1 < iso && d0 >= iso;  // type a + 3 (src/cudualmc.cu:782-788)
+            L.quad[q++] = (uint16_t)(lane * 8 + a * 2 + leaving);
+        }
+    }
+    wave_lds_sync();
+
+    const int64_t vlim = tot[0].v;  // MC vertex count (rows of mcv)
+    for (int k = lane; k < tp; k += kWave) {
+        const int it = L.patch[k], src = it >> 2, pk = it & 3;
+        unsigned em = T.pedges[L.code[0][src]][pk];
+        const int num = __popc(em);
+        S sx = S(0), sy = S(0), sz = S(0);
+        while (em) {  // ascending edge order
+            const int e = __ffs(em) - 1;
+            em &= em - 1;
+            const int64_t v = qbase + L.cand[src][e];
+            if (v >= 0 && v < vlim) {
+                sx += mcv[3 * v + 0];
+                sy += mcv[3 * v + 1];
+                sz += mcv[3 * v + 2];
             }
-            if (q < out.fcap) {
+        }
+        const S r = T.rcp[num];
+        store_vertex(verts, pbase + k, out.vcap, epilogue(V3<S>{sx * r, sy * r, sz * r}, out.flags, nm));
+        if (pbase + k < out.vcap) vinfo[pbase + k] = (uint8_t)(num | (pk << 3));
+    }
+    for (int k = lane; k < tq; k += kWave) {
+        const int it = L.quad[k], src = it >> 3, ty = ((it >> 1) & 3) + 3 * (it & 1);
+        const int64_t q = qbase + k;
+        if (q >= out.fcap) continue;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) store_index(faces, 4 * q + k, id[k], out.flags);
-            }
-            ++q;
+        for (int j = 0; j < 4; ++j) {
+            const int ci = T.st_ci[ty][j], eid = T.st_eid[ty][j];
+            const int p = DISO_DMC_PATCH_OF(T.cases[L.code[ci][src]], eid);
+            // dmcEdgeOffset is -1 for a non-crossing edge (never the case on a padded grid)
+            store_index(faces, 4 * q + j, pbase + L.base[ci][src] + (p == 7 ? -1 : p), out.flags);
         }
     }
 }
 
 // ------------------------------------------------------------------------------------------
 // Backward (gather, deterministic)
-
-template <typename S>
-struct Grad {
-    S g, dx, dy, dz;
-};
-
-// Contribution of one crossing edge (v0 -> v0 + e_axis, upstream gradient gv) to the endpoint
-// `end` (0 = v0, 1 = v1): adjComputeMcVert (src/cumc.cu:412-453).
-template <typename S, bool DEF>
-__device__ __forceinline__ void edge_adjoint(const In<S> &in, int x, int y, int z, int axis, int end,
-                                             V3<S> gv, Grad<S> *acc) {
-    const int x1 = x + (axis == 0), y1 = y + (axis == 1), z1 = z + (axis == 2);
-    const S d0 = sample(in, x, y, z), d1 = sample(in, x1, y1, z1);
-    const S t = interp_t(d0, d1, in.iso);
-    S p0x = S(x), p0y = S(y), p0z = S(z);
-    S p1x = S(x1), p1y = S(y1), p1z = S(z1);
-    if (DEF) {
-        V3<S> a = deform_at<S, DEF>(in, x, y, z), b = deform_at<S, DEF>(in, x1, y1, z1);
-        p0x += a.x;
-        p0y += a.y;
-        p0z += a.z;
-        p1x += b.x;
-        p1y += b.y;
-        p1z += b.z;
-    }
-    const S adj_t = (p1x - p0x) * gv.x + (p1y - p0y) * gv.y + (p1z - p0z) * gv.z;
-    const S dd = (d1 - d0) * (d1 - d0);
-    if (end == 0) {
-        acc->g += (in.iso - d1) / dd * adj_t;
-        const S w = S(1) - t;
-        acc->dx += w * gv.x;
-        acc->dy += w * gv.y;
-        acc->dz += w * gv.z;
-    } else {
-        acc->g += (d0 - in.iso) / dd * adj_t;
-        acc->dx += t * gv.x;
-        acc->dy += t * gv.y;
-        acc->dz += t * gv.z;
-    }
-}
+//
+// MC: one lane per grid vertex; its incident crossing edges (at most six: the three it owns and
+// the ones owned by its -x, -y, -z neighbours) become items of a dense per-wave list. Each item
+// computes the adjoint of its edge vertex (adjComputeMcVert, src/cumc.cu:412-453) for the lane's
+// endpoint; the owning lane adds its items up in a fixed order (slot 0..5), so results are
+// deterministic and no atomics are used. Item slots: 0/1/2 = edge along x/y/z owned by the
+// vertex (vertex is its start), 3/4/5 = edge along x/y/z owned by the -x/-y/-z neighbour
+// (vertex is its end).
+//
+// DMC: dmc_edge_grad_kernel first turns the DMC vertex gradients into one upstream gradient per
+// crossing edge (MC vertex order = quad order): the sum over the quad's four corners of (gradient
+// of that patch vertex) / (edges in its patch), the adjoint of create_dmc_verts_kernel's mean
+// (src/cudualmc.cu:907-955); with DISO_FLAG_REFERENCE_COMPAT each corner reads its cell's first
+// patch instead (src/cudualmc.cu:990). The MC gather then runs on those edge gradients.
 
 template <typename S>
 struct AdjIn {
-    const S *adj;  // [n_verts, 3]
+    const S *__restrict__ adj;  // [n, 3]
     int64_t n;
     int flags;
-    S nx, ny, nz;  // upstream scaling by 1/(dims-1) when the forward normalised
+    Norm<S> nm;  // upstream scaling by 1/(dims-1) when the forward normalised
 };
 
 template <typename S>
@@ -805,18 +991,69 @@ __device__ __forceinline__ V3<S> load_adj(const AdjIn<S> &a, int64_t i) {
         r.x = a.adj[3 * i + 0];
         r.y = a.adj[3 * i + 1];
         r.z = a.adj[3 * i + 2];
-        if (a.flags & DISO_FLAG_NORMALIZE) {  // grad of verts / (dims - 1)
-            r.x = r.x / a.nx;
-            r.y = r.y / a.ny;
-            r.z = r.z / a.nz;
+        if (a.flags & DISO_FLAG_NORMALIZE) {  // grad of verts / (dims - 1): g / (dims - 1)
+            r.x = div_by(r.x, a.nm.c[0], a.nm.r[0]);
+            r.y = div_by(r.y, a.nm.c[1], a.nm.r[1]);
+            r.z = div_by(r.z, a.nm.c[2], a.nm.r[2]);
         }
     }
     return r;
 }
 
 template <typename S>
-__device__ __forceinline__ void store_grad(const In<S> &in, S *adj_sdf, S *adj_deform, int x, int y,
-                                           int z, Grad<S> acc, int flags) {
+struct Grad {
+    S g, dx, dy, dz;
+};
+
+// d t / d d0 and d t / d d1 factors of t = (iso - d0) / (d1 - d0), and t itself.
+template <typename S>
+__device__ __forceinline__ void interp_adjoint(S d0, S d1, S iso, S *t, S *k0, S *k1) {
+    *t = interp_t(d0, d1, iso);
+    const S dd = (d1 - d0) * (d1 - d0);
+    *k0 = (iso - d1) / dd;
+    *k1 = (d0 - iso) / dd;
+}
+// fp32: gradients are compared within a tolerance, so one hardware reciprocal replaces the three
+// correctly rounded divisions (a few ulp).
+template <>
+__device__ __forceinline__ void interp_adjoint<float>(float d0, float d1, float iso, float *t, float *k0, float *k1) {
+    const float r = __builtin_amdgcn_rcpf(d1 - d0);
+    *t = d1 != d0 ? clamp01<float>((iso - d0) * r) : 0.5f;
+    const float r2 = r * r;
+    *k0 = (iso - d1) * r2;
+    *k1 = (d0 - iso) * r2;
+}
+
+// Adjoint of one edge vertex (v0 -> v0 + e_axis, upstream gradient gv) at endpoint `end`
+// (0 = v0, 1 = v1), with the endpoints' samples d0, d1 and deforms f0, f1 (src/cumc.cu:412-453).
+template <typename S, bool DEF>
+__device__ __forceinline__ Grad<S> edge_adjoint(int x, int y, int z, int axis, int end, S d0, S d1, V3<S> f0,
+                                                V3<S> f1, S iso, V3<S> gv) {
+    S t, k0, k1;
+    interp_adjoint(d0, d1, iso, &t, &k0, &k1);
+    S p0x = S(x), p0y = S(y), p0z = S(z);
+    S p1x = S(x + (axis == 0)), p1y = S(y + (axis == 1)), p1z = S(z + (axis == 2));
+    if (DEF) {
+        p0x += f0.x;
+        p0y += f0.y;
+        p0z += f0.z;
+        p1x += f1.x;
+        p1y += f1.y;
+        p1z += f1.z;
+    }
+    const S adj_t = (p1x - p0x) * gv.x + (p1y - p0y) * gv.y + (p1z - p0z) * gv.z;
+    const S wgt = end ? t : S(1) - t;
+    Grad<S> r;
+    r.g = (end ? k1 : k0) * adj_t;
+    r.dx = wgt * gv.x;
+    r.dy = wgt * gv.y;
+    r.dz = wgt * gv.z;
+    return r;
+}
+
+template <typename S>
+__device__ __forceinline__ void store_grad(const In<S> &in, S *__restrict__ adj_sdf, S *__restrict__ adj_deform, int x,
+                                           int y, int z, Grad<S> acc, int flags) {
     if (!in_data(in, x, y, z)) return;
     const int64_t i = data_index(in, x, y, z);
     if (flags & DISO_FLAG_ACCUMULATE) {
@@ -836,12 +1073,23 @@ __device__ __forceinline__ void store_grad(const In<S> &in, S *adj_sdf, S *adj_d
     }
 }
 
-// MC: the grid vertex v gathers the edges v->v+e_a (owned by v, ids base(v)+rank) and
-// v-e_a->v (owned by v-e_a: the -x owner's x edge is its first vertex, the -y owner's y edge is
-// its first vertex plus its x flag, the -z owner's z edge is the vertex just before base(v)).
+// Per-wave LDS of the gather: the samples a vertex needs, its own (0), +x +y +z (1..3),
+// -x -y -z (4..6); its deform; the vertex id of each incident edge relative to the segment's
+// first vertex; the item list (lane * 8 + slot) and one round of item results.
+template <typename S>
+struct BwdLds {
+    S smp[7][kWave];
+    S fc[3][kWave];
+    int32_t id[6][kWave];
+    uint16_t item[kWave * 6];
+    Grad<S> res[kWave];
+};
+
 template <typename S, bool DEF>
-__global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase *seg, AdjIn<S> adj,
-                                                        S *adj_sdf, S *adj_deform, int64_t nblocks) {
+__global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase *__restrict__ seg, AdjIn<S> adj,
+                                                        S *__restrict__ adj_sdf, S *__restrict__ adj_deform,
+                                                        int64_t nblocks) {
+    __shared__ BwdLds<S> s_lds[kWavesPerBlock];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
     const Geo &g = in.geo;
     const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
@@ -849,118 +1097,123 @@ __global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase 
     const int chunk = (int)(s % g.nchunk);
     const int64_t row = s / g.nchunk;
     const int px = (int)(row / g.g1), py = (int)(row % g.g1);
-    const int pz = chunk * kWave + lane;
+    const int z0 = chunk * kWave, pz = z0 + lane;
+    const S iso = in.iso;
+    const int64_t vbase = seg[s].v;
+    const int64_t bm0 = px > 0 ? seg[seg_index(g, px - 1, py, chunk)].v : 0;
+    const int64_t b0m = py > 0 ? seg[seg_index(g, px, py - 1, chunk)].v : 0;
 
-    const int m00 = owned_crossings(in, px, py, pz);
-    const int mm0 = owned_crossings(in, px - 1, py, pz);
-    const int m0m = owned_crossings(in, px, py - 1, pz);
-    const int m00m = owned_crossings(in, px, py, pz - 1);
-    int t;
-    const int64_t vb00 = seg[s].v + wave_excl_small(popc3(m00), &t);
-    const int64_t vbm0 = (px > 0 ? seg[seg_index(g, px - 1, py, chunk)].v : 0) + wave_excl_small(popc3(mm0), &t);
-    const int64_t vb0m = (py > 0 ? seg[seg_index(g, px, py - 1, chunk)].v : 0) + wave_excl_small(popc3(m0m), &t);
-    if (!in_data(in, px, py, pz)) return;
-
+    S smp[7];
+    smp[0] = sample(in, px, py, pz);
+    smp[1] = sample(in, px + 1, py, pz);
+    smp[2] = sample(in, px, py + 1, pz);
+    smp[3] = sample(in, px, py, pz + 1);
+    smp[4] = sample(in, px - 1, py, pz);
+    smp[5] = sample(in, px, py - 1, pz);
+    smp[6] = sample(in, px, py, pz - 1);
+    const S c = smp[0];
+    const int m00 = is_anchor(in, px, py, pz) ? own_mask(c, smp[1], smp[2], smp[3], iso) : 0;
+    const int mm0 = is_anchor(in, px - 1, py, pz)
+                        ? own_mask(smp[4], c, sample(in, px - 1, py + 1, pz), sample(in, px - 1, py, pz + 1), iso)
+                        : 0;
+    const int m0m = is_anchor(in, px, py - 1, pz)
+                        ? own_mask(smp[5], sample(in, px + 1, py - 1, pz), c, sample(in, px, py - 1, pz + 1), iso)
+                        : 0;
+    const int m00m = is_anchor(in, px, py, pz - 1)
+                         ? own_mask(smp[6], sample(in, px + 1, py, pz - 1), sample(in, px, py + 1, pz - 1), c, iso)
+                         : 0;
+    const int mask = in_data(in, px, py, pz) ? (m00 | ((mm0 & 1) << 3) | ((m0m & 2) << 3) | ((m00m & 4) << 3)) : 0;
     Grad<S> acc = {S(0), S(0), S(0), S(0)};
-    int r = 0;
-    if (m00 & 1) edge_adjoint<S, DEF>(in, px, py, pz, 0, 0, load_adj(adj, vb00 + r++), &acc);
-    if (m00 & 2) edge_adjoint<S, DEF>(in, px, py, pz, 1, 0, load_adj(adj, vb00 + r++), &acc);
-    if (m00 & 4) edge_adjoint<S, DEF>(in, px, py, pz, 2, 0, load_adj(adj, vb00 + r++), &acc);
-    if (mm0 & 1) edge_adjoint<S, DEF>(in, px - 1, py, pz, 0, 1, load_adj(adj, vbm0), &acc);
-    if (m0m & 2) edge_adjoint<S, DEF>(in, px, py - 1, pz, 1, 1, load_adj(adj, vb0m + (m0m & 1)), &acc);
-    if (m00m & 4) edge_adjoint<S, DEF>(in, px, py, pz - 1, 2, 1, load_adj(adj, vb00 - 1), &acc);
-    store_grad(in, adj_sdf, adj_deform, px, py, pz, acc, adj.flags);
-}
-
-// DMC: as MC, but an edge's upstream gradient is the sum over the (up to four) cells around it
-// of (gradient of the patch vertex holding the edge) / (edges in that patch), i.e. the adjoint
-// of create_dmc_verts_kernel's mean. With DISO_FLAG_REFERENCE_COMPAT each cell contributes its
-// first patch's gradient instead (src/cudualmc.cu:990).
-template <typename S>
-__device__ __forceinline__ S recip_count(int n) {
-    switch (n) {  // S(1)/S(n), correctly rounded
-        case 1: return S(1) / S(1);
-        case 2: return S(1) / S(2);
-        case 3: return S(1) / S(3);
-        case 4: return S(1) / S(4);
-        case 5: return S(1) / S(5);
-        case 6: return S(1) / S(6);
-        default: return S(1) / S(7);
+    if (__ballot(mask != 0) == 0) {  // no crossing edge touches this segment: zero gradients
+        store_grad(in, adj_sdf, adj_deform, px, py, pz, acc, adj.flags);
+        return;
     }
-}
+    int t, total;
+    const int32_t r00 = wave_excl_small(popc3(m00), &t);
+    const int32_t rm0 = (int32_t)(bm0 - vbase) + wave_excl_small(popc3(mm0), &t);
+    const int32_t r0m = (int32_t)(b0m - vbase) + wave_excl_small(popc3(m0m), &t);
+    const int first = wave_excl_small(__popc(mask), &total);
 
-__device__ __forceinline__ int patch_size(uint64_t w, int p) {
-    int n = 0;
+    BwdLds<S> &L = s_lds[wave];
 #pragma unroll
-    for (int e = 0; e < 12; ++e) n += DISO_DMC_PATCH_OF(w, e) == p;
-    return n;
-}
+    for (int i = 0; i < 7; ++i) L.smp[i][lane] = smp[i];
+    if (mask) {
+        const V3<S> f = deform_at<S, DEF>(in, px, py, pz);
+        L.fc[0][lane] = f.x;
+        L.fc[1][lane] = f.y;
+        L.fc[2][lane] = f.z;
+    }
+    L.id[0][lane] = r00;
+    L.id[1][lane] = r00 + (m00 & 1);
+    L.id[2][lane] = r00 + (m00 & 1) + ((m00 >> 1) & 1);
+    L.id[3][lane] = rm0;
+    L.id[4][lane] = r0m + (m0m & 1);
+    L.id[5][lane] = r00 - 1;
+    {
+        int r = first;
+#pragma unroll
+        for (int sl = 0; sl < 6; ++sl)
+            if ((mask >> sl) & 1) L.item[r++] = (uint16_t)(lane * 8 + sl);
+    }
+    wave_lds_sync();
 
-template <typename S>
-__device__ __forceinline__ V3<S> dmc_cell_adjoint(const uint64_t *cases, const AdjIn<S> &adj,
-                                                  const DmcCell &c, int eid, bool compat) {
-    const uint64_t w = cases[c.code];
-    const int p = DISO_DMC_PATCH_OF(w, eid);
-    V3<S> r = {S(0), S(0), S(0)};
-    if (p == 7) return r;
-    V3<S> a = load_adj(adj, compat ? c.base : c.base + p);
-    const S k = recip_count<S>(patch_size(w, p));
-    r.x = a.x * k;
-    r.y = a.y * k;
-    r.z = a.z * k;
-    return r;
-}
-
-template <typename S>
-__device__ __forceinline__ V3<S> add4(V3<S> a, V3<S> b, V3<S> c, V3<S> d) {
-    V3<S> r;
-    r.x = ((a.x + b.x) + c.x) + d.x;
-    r.y = ((a.y + b.y) + c.y) + d.y;
-    r.z = ((a.z + b.z) + c.z) + d.z;
-    return r;
-}
-
-template <typename S, bool DEF>
-__global__ __launch_bounds__(kBlock) void bwd_dmc_kernel(In<S> in, const SegBase *seg, AdjIn<S> adj,
-                                                         S *adj_sdf, S *adj_deform, int64_t nblocks) {
-    __shared__ uint64_t s_case[256];
-    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
-    for (int i = tid; i < 256; i += kBlock) s_case[i] = kDmcCase[i];
-    __syncthreads();
-    const Geo &g = in.geo;
-    const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
-    if (s >= g.nseg) return;
-    const int chunk = (int)(s % g.nchunk);
-    const int64_t row = s / g.nchunk;
-    const int px = (int)(row / g.g1), py = (int)(row % g.g1);
-    const int pz = chunk * kWave + lane;
-    const bool compat = (adj.flags & DISO_FLAG_REFERENCE_COMPAT) != 0;
-
-    DmcCell c00, c00m, c0m, c0mm, cm0, cm0m, cmm, cmmm;
-    dmc_row(in, s_case, seg, px, py, chunk, pz, &c00, &c00m);
-    dmc_row(in, s_case, seg, px, py - 1, chunk, pz, &c0m, &c0mm);
-    dmc_row(in, s_case, seg, px - 1, py, chunk, pz, &cm0, &cm0m);
-    dmc_row(in, s_case, seg, px - 1, py - 1, chunk, pz, &cmm, &cmmm);
-    if (!in_data(in, px, py, pz)) return;
-
-    const int m00 = owned_crossings(in, px, py, pz);
-    const int mm0 = owned_crossings(in, px - 1, py, pz);
-    const int m0m = owned_crossings(in, px, py - 1, pz);
-    const int m00m = owned_crossings(in, px, py, pz - 1);
-    Grad<S> acc = {S(0), S(0), S(0), S(0)};
-#define DMC_EDGE_GRAD(C0, E0, C1, E1, C2, E2, C3, E3)                                               \
-    add4(dmc_cell_adjoint(s_case, adj, C0, E0, compat), dmc_cell_adjoint(s_case, adj, C1, E1, compat), \
-         dmc_cell_adjoint(s_case, adj, C2, E2, compat), dmc_cell_adjoint(s_case, adj, C3, E3, compat))
-    // Cells around each incident edge, in the order of the reference's quad stencil for the
-    // edge's axis (src/cudualmc.cu:504-507).
-    if (m00 & 1) edge_adjoint<S, DEF>(in, px, py, pz, 0, 0, DMC_EDGE_GRAD(c00, 0, c0m, 4, c0mm, 6, c00m, 2), &acc);
-    if (m00 & 2) edge_adjoint<S, DEF>(in, px, py, pz, 1, 0, DMC_EDGE_GRAD(c00, 8, c00m, 11, cm0m, 10, cm0, 9), &acc);
-    if (m00 & 4) edge_adjoint<S, DEF>(in, px, py, pz, 2, 0, DMC_EDGE_GRAD(c00, 3, cm0, 1, cmm, 5, c0m, 7), &acc);
-    if (mm0 & 1) edge_adjoint<S, DEF>(in, px - 1, py, pz, 0, 1, DMC_EDGE_GRAD(cm0, 0, cmm, 4, cmmm, 6, cm0m, 2), &acc);
-    if (m0m & 2) edge_adjoint<S, DEF>(in, px, py - 1, pz, 1, 1, DMC_EDGE_GRAD(c0m, 8, c0mm, 11, cmmm, 10, cmm, 9), &acc);
-    if (m00m & 4) edge_adjoint<S, DEF>(in, px, py, pz - 1, 2, 1, DMC_EDGE_GRAD(c00m, 3, cm0m, 1, cmmm, 5, c0mm, 7), &acc);
-#undef DMC_EDGE_GRAD
+    const int last = first + __popc(mask);
+    for (int b = 0; b < total; b += kWave) {
+        const int k = b + lane;
+        if (k < total) {
+            const int it = L.item[k], src = it >> 3, slot = it & 7;
+            const int axis = slot % 3, end = slot >= 3;
+            const int x = px - (slot == 3), y = py - (slot == 4), z = z0 + src - (slot == 5);
+            const S d0 = slot < 3 ? L.smp[0][src] : L.smp[slot + 1][src];
+            const S d1 = slot < 3 ? L.smp[slot + 1][src] : L.smp[0][src];
+            const V3<S> fv = {L.fc[0][src], L.fc[1][src], L.fc[2][src]};
+            // the edge's other endpoint
+            const int ex = end ? x : x + (axis == 0), ey = end ? y : y + (axis == 1), ez = end ? z : z + (axis == 2);
+            const V3<S> fo = deform_at<S, DEF>(in, ex, ey, ez);
+            L.res[lane] = edge_adjoint<S, DEF>(x, y, z, axis, end, d0, d1, end ? fo : fv, end ? fv : fo, iso,
+                                               load_adj(adj, vbase + L.id[slot][src]));
+        }
+        wave_lds_sync();
+        const int lo = first > b ? first : b, hi = last < b + kWave ? last : b + kWave;
+        for (int j = lo; j < hi; ++j) {  // this lane's items of the round, in slot order
+            const Grad<S> r = L.res[j - b];
+            acc.g += r.g;
+            acc.dx += r.dx;
+            acc.dy += r.dy;
+            acc.dz += r.dz;
+        }
+        wave_lds_sync();
+    }
     store_grad(in, adj_sdf, adj_deform, px, py, pz, acc, adj.flags);
+}
+
+// Per-edge upstream gradient of a DMC forward: ge[q] = sum over the quad's corners v of
+// (dL/dv) / (edges in v's patch), corners in the quad's order.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void dmc_edge_grad_kernel(const void *__restrict__ quads, int64_t nq,
+                                                               const uint8_t *__restrict__ vinfo, AdjIn<S> adj,
+                                                               S *__restrict__ ge) {
+    __shared__ S s_rcp[8];
+    if (threadIdx.x < 8) s_rcp[threadIdx.x] = threadIdx.x ? S(1) / S(threadIdx.x) : S(0);
+    __syncthreads();
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= nq) return;
+    const bool compat = (adj.flags & DISO_FLAG_REFERENCE_COMPAT) != 0;
+    V3<S> sum = {S(0), S(0), S(0)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t v = load_index(quads, 4 * q + k, adj.flags);
+        if (v < 0 || v >= adj.n) continue;
+        const int info = vinfo[v];
+        const V3<S> a = load_adj(adj, compat ? v - (info >> 3) : v);
+        const S r = s_rcp[info & 7];
+        sum.x += a.x * r;
+        sum.y += a.y * r;
+        sum.z += a.z * r;
+    }
+    ge[3 * q + 0] = sum.x;
+    ge[3 * q + 1] = sum.y;
+    ge[3 * q + 2] = sum.z;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1051,64 +1304,80 @@ int launch_count(int algo, const void *sdf, double iso, const Geo &g, void *seg,
 int64_t row_blocks(const Geo &g) { return (g.nseg + kWavesPerBlock - 1) / kWavesPerBlock; }
 
 template <typename S>
-void norm_of(const Geo &g, S *nx, S *ny, S *nz) {
-    *nx = (S)g.n0 - S(1);
-    *ny = (S)g.n1 - S(1);
-    *nz = (S)g.n2 - S(1);
+Norm<S> norm_of(const Geo &g) {
+    Norm<S> nm;
+    const int n[3] = {g.n0, g.n1, g.n2};
+    for (int a = 0; a < 3; ++a) {
+        nm.c[a] = (S)n[a] - S(1);
+        nm.r[a] = S(1) / nm.c[a];
+    }
+    return nm;
 }
 
 template <typename S>
 int launch_emit(int algo, const void *sdf, const void *deform, double iso, int flags, const Geo &g,
-                const void *seg, void *verts, int64_t vcap, void *faces, int64_t fcap,
-                hipStream_t st) {
+                const void *seg, void *verts, int64_t vcap, void *faces, int64_t fcap, void *vinfo, void *scratch,
+                int64_t scratch_bytes, hipStream_t st) {
     In<S> in = make_in<S>(sdf, deform, iso, g);
-    OutCfg out{flags, vcap, fcap};
-    S nx, ny, nz;
-    norm_of(g, &nx, &ny, &nz);
+    const Norm<S> nm = norm_of<S>(g);
     const int64_t nb = row_blocks(g);
     const SegBase *sb = (const SegBase *)seg;
+    const OutCfg out{flags, vcap, fcap};
     if (algo == DISO_MC) {
         if (deform)
-            hipLaunchKernelGGL((emit_mc_kernel<S, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
-                               (S *)verts, faces, out, nx, ny, nz, nb);
+            hipLaunchKernelGGL((emit_mc_kernel<S, true, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
+                               (S *)verts, faces, out, nm, nb);
         else
-            hipLaunchKernelGGL((emit_mc_kernel<S, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
-                               (S *)verts, faces, out, nx, ny, nz, nb);
-    } else {
-        if (deform)
-            hipLaunchKernelGGL((emit_dmc_kernel<S, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
-                               (S *)verts, faces, out, nx, ny, nz, nb);
-        else
-            hipLaunchKernelGGL((emit_dmc_kernel<S, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
-                               (S *)verts, faces, out, nx, ny, nz, nb);
+            hipLaunchKernelGGL((emit_mc_kernel<S, false, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
+                               (S *)verts, faces, out, nm, nb);
+        return check_hip(hipGetLastError(), "emit kernel launch");
     }
+    // DMC: MC edge vertices into scratch (no epilogue), then patch means and quads.
+    const int64_t mcap = scratch_bytes / (3 * (int64_t)sizeof(S));
+    const OutCfg mout{0, mcap, 0};
+    if (deform)
+        hipLaunchKernelGGL((emit_mc_kernel<S, true, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
+                           (S *)scratch, nullptr, mout, nm, nb);
+    else
+        hipLaunchKernelGGL((emit_mc_kernel<S, false, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
+                           (S *)scratch, nullptr, mout, nm, nb);
+    int rc = check_hip(hipGetLastError(), "edge vertex kernel launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL((emit_dmc_kernel<S>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb, (const S *)scratch, mcap,
+                       (S *)verts, faces, (uint8_t *)vinfo, out, nm, nb);
     return check_hip(hipGetLastError(), "emit kernel launch");
 }
 
 template <typename S>
-int launch_backward(int algo, const void *sdf, const void *deform, double iso, int flags,
-                    const Geo &g, const void *seg, const void *adj_verts, int64_t nverts,
-                    void *adj_sdf, void *adj_deform, hipStream_t st) {
+int launch_backward(int algo, const void *sdf, const void *deform, double iso, int flags, const Geo &g,
+                    const void *seg, const void *faces, int64_t nfaces, const void *vinfo, const void *adj_verts,
+                    int64_t nverts, void *adj_sdf, void *adj_deform, void *scratch, hipStream_t st) {
     In<S> in = make_in<S>(sdf, deform, iso, g);
     AdjIn<S> a;
     a.adj = (const S *)adj_verts;
     a.n = nverts;
     a.flags = flags;
-    norm_of(g, &a.nx, &a.ny, &a.nz);
+    a.nm = norm_of<S>(g);
     const int64_t nb = row_blocks(g);
     const SegBase *sb = (const SegBase *)seg;
     S *gs = (S *)adj_sdf, *gd = (S *)(deform ? adj_deform : nullptr);
-    if (algo == DISO_MC) {
-        if (deform)
-            hipLaunchKernelGGL((bwd_mc_kernel<S, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb, a, gs, gd, nb);
-        else
-            hipLaunchKernelGGL((bwd_mc_kernel<S, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb, a, gs, gd, nb);
-    } else {
-        if (deform)
-            hipLaunchKernelGGL((bwd_dmc_kernel<S, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb, a, gs, gd, nb);
-        else
-            hipLaunchKernelGGL((bwd_dmc_kernel<S, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb, a, gs, gd, nb);
+    if (algo == DISO_DMC) {
+        // per-edge upstream gradients from the quads, then the MC gather on them
+        if (nfaces > 0) {
+            const int64_t eb = (nfaces + kBlock - 1) / kBlock;
+            hipLaunchKernelGGL((dmc_edge_grad_kernel<S>), dim3((unsigned)eb), dim3(kBlock), 0, st, faces, nfaces,
+                               (const uint8_t *)vinfo, a, (S *)scratch);
+            int rc = check_hip(hipGetLastError(), "edge gradient kernel launch");
+            if (rc) return rc;
+        }
+        a.adj = (const S *)scratch;
+        a.n = nfaces;
+        a.flags = flags & ~DISO_FLAG_NORMALIZE;  // applied by the edge gradient kernel
     }
+    if (deform)
+        hipLaunchKernelGGL((bwd_mc_kernel<S, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb, a, gs, gd, nb);
+    else
+        hipLaunchKernelGGL((bwd_mc_kernel<S, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb, a, gs, gd, nb);
     return check_hip(hipGetLastError(), "backward kernel launch");
 }
 
@@ -1198,9 +1467,17 @@ int diso_read_totals(const void *seg, int64_t seg_bytes, int64_t X, int64_t Y, i
     return read_totals(g, seg, totals_host, (hipStream_t)stream);
 }
 
-int diso_emit(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y,
-              int64_t Z, double iso, int flags, const void *seg, int64_t seg_bytes, void *verts,
-              int64_t vert_capacity, void *faces, int64_t face_capacity, void *stream) {
+int64_t diso_scratch_bytes(int algo, int dtype, int64_t n_crossing) {
+    if ((algo != DISO_MC && algo != DISO_DMC) || (dtype != DISO_F32 && dtype != DISO_F64) || n_crossing < 0)
+        return fail(DISO_E_INVALID, "bad scratch query");
+    if (algo == DISO_MC) return 0;
+    return n_crossing * 3 * (dtype == DISO_F32 ? 4 : 8);
+}
+
+int diso_emit(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y, int64_t Z,
+              double iso, int flags, const void *seg, int64_t seg_bytes, void *verts, int64_t vert_capacity,
+              void *faces, int64_t face_capacity, void *vinfo, void *scratch, int64_t scratch_bytes,
+              void *stream) {
     g_err[0] = 0;
     Geo g;
     int rc = check_common(algo, dtype, X, Y, Z, flags, &g);
@@ -1209,18 +1486,22 @@ int diso_emit(int algo, int dtype, const void *sdf, const void *deform, int64_t 
     if (seg_bytes < seg_bytes_for(g)) return fail(DISO_E_CAPACITY, "seg workspace too small");
     if (vert_capacity < 0 || face_capacity < 0) return fail(DISO_E_INVALID, "negative capacity");
     if ((vert_capacity && !verts) || (face_capacity && !faces)) return fail(DISO_E_INVALID, "null output buffer");
+    if (algo == DISO_DMC) {
+        if (vert_capacity && !vinfo) return fail(DISO_E_INVALID, "DMC needs a vinfo buffer");
+        if (scratch_bytes < diso_scratch_bytes(algo, dtype, face_capacity) || (face_capacity && !scratch))
+            return fail(DISO_E_CAPACITY, "scratch buffer too small for %lld quads", (long long)face_capacity);
+    }
     hipStream_t st = (hipStream_t)stream;
-    return dtype == DISO_F32
-               ? launch_emit<float>(algo, sdf, deform, iso, flags, g, seg, verts, vert_capacity, faces,
-                                    face_capacity, st)
-               : launch_emit<double>(algo, sdf, deform, iso, flags, g, seg, verts, vert_capacity, faces,
-                                     face_capacity, st);
+    return dtype == DISO_F32 ? launch_emit<float>(algo, sdf, deform, iso, flags, g, seg, verts, vert_capacity, faces,
+                                                  face_capacity, vinfo, scratch, scratch_bytes, st)
+                             : launch_emit<double>(algo, sdf, deform, iso, flags, g, seg, verts, vert_capacity, faces,
+                                                   face_capacity, vinfo, scratch, scratch_bytes, st);
 }
 
-int diso_backward(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y,
-                  int64_t Z, double iso, int flags, const void *seg, int64_t seg_bytes,
-                  const void *adj_verts, int64_t n_verts, void *adj_sdf, void *adj_deform,
-                  void *stream) {
+int diso_backward(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y, int64_t Z,
+                  double iso, int flags, const void *seg, int64_t seg_bytes, const void *faces, int64_t n_faces,
+                  const void *vinfo, const void *adj_verts, int64_t n_verts, void *adj_sdf, void *adj_deform,
+                  void *scratch, int64_t scratch_bytes, void *stream) {
     g_err[0] = 0;
     Geo g;
     int rc = check_common(algo, dtype, X, Y, Z, flags, &g);
@@ -1229,11 +1510,16 @@ int diso_backward(int algo, int dtype, const void *sdf, const void *deform, int6
     if (deform && !adj_deform) return fail(DISO_E_INVALID, "adj_deform required when deform is given");
     if (n_verts < 0 || (n_verts && !adj_verts)) return fail(DISO_E_INVALID, "bad adj_verts");
     if (seg_bytes < seg_bytes_for(g)) return fail(DISO_E_CAPACITY, "seg workspace too small");
+    if (algo == DISO_DMC) {
+        if (n_faces < 0 || (n_faces && (!faces || !vinfo))) return fail(DISO_E_INVALID, "DMC backward needs the quads and vinfo");
+        if (scratch_bytes < diso_scratch_bytes(algo, dtype, n_faces) || (n_faces && !scratch))
+            return fail(DISO_E_CAPACITY, "scratch buffer too small for %lld quads", (long long)n_faces);
+    }
     hipStream_t st = (hipStream_t)stream;
-    return dtype == DISO_F32 ? launch_backward<float>(algo, sdf, deform, iso, flags, g, seg, adj_verts,
-                                                      n_verts, adj_sdf, adj_deform, st)
-                             : launch_backward<double>(algo, sdf, deform, iso, flags, g, seg, adj_verts,
-                                                       n_verts, adj_sdf, adj_deform, st);
+    return dtype == DISO_F32 ? launch_backward<float>(algo, sdf, deform, iso, flags, g, seg, faces, n_faces, vinfo,
+                                                      adj_verts, n_verts, adj_sdf, adj_deform, scratch, st)
+                             : launch_backward<double>(algo, sdf, deform, iso, flags, g, seg, faces, n_faces, vinfo,
+                                                       adj_verts, n_verts, adj_sdf, adj_deform, scratch, st);
 }
 
 }  // extern "C"

@@ -103,72 +103,73 @@ static const uint64_t DISO_MC_CASE[256] = {DISO_MC_CASE_VALUES};
 // DMC case word: bits [3e, 3e+3) = patch index (0..3) that crossing edge e belongs to, 7 when
 // edge e does not cross; bits [36,39) = patch count (0..4); bits [39,42) = ambiguity direction
 // d (0..5: face neighbour along axis d>>1, towards + when d&1) or 7 when the case is not
-// ambiguous. Within a patch the edges are taken in ascending edge id.
+// ambiguous; bits [42+3k, 45+3k) = number of edges in patch k (derived from the edge fields).
+// Within a patch the edges are taken in ascending edge id.
 #define DISO_DMC_CASE_VALUES \
-    0x0000038fffffffffull, 0x0000039ff8fff1f8ull, 0x0000039fc7ffffc0ull, 0x0000039fc0fff1c7ull, \
-    0x0000039ff81f8fffull, 0x0000039fff1f81f8ull, 0x000003afc13f9fc0ull, 0x0000039fc71f81c7ull, \
-    0x0000039fc7fc0fffull, 0x000003afc8fc91f8ull, 0x0000039ffffc0fc0ull, 0x0000039ff8fc01c7ull, \
-    0x0000039fc01c7fffull, 0x0000039fc71c71f8ull, 0x0000039ff81c7fc0ull, 0x0000039fff1c71c7ull, \
-    0x00000391fffff03full, 0x00000391f8fffe38ull, 0x000003a3c7fff240ull, 0x00000391c0fffe07ull, \
-    0x000003a3f81f827full, 0x00000391ff1f8e38ull, 0x000003b5c13f9480ull, 0x00000391c71f8e07ull, \
-    0x000003a3c7fc027full, 0x000003a1c8fc9e38ull, 0x000003a3fffc0240ull, 0x00000391f8fc0e07ull, \
-    0x000003a3c01c727full, 0x00000391c71c7e38ull, 0x000003a3f81c7240ull, 0x00000391ff1c7e07ull, \
-    0x0000039e3ffffe07ull, 0x000003ae78fff048ull, 0x0000039e07fffe38ull, 0x0000039e00fff03full, \
-    0x000003ae781f8e4full, 0x000003ae7f1f8048ull, 0x000003ae013f9e38ull, 0x0000039e071f803full, \
-    0x000003ae47fc0e4full, 0x000003be88fc9090ull, 0x0000039e3ffc0e38ull, 0x0000039e38fc003full, \
-    0x000003ae401c7e4full, 0x000003ae471c7048ull, 0x0000039e381c7e38ull, 0x0000039e3f1c703full, \
-    0x000003903ffff1c7ull, 0x0000039038ffffc0ull, 0x0000039007fff1f8ull, 0x0000039000ffffffull, \
-    0x000003a2781f83cfull, 0x000003903f1f8fc0ull, 0x000003a0013f91f8ull, 0x00000390071f8fffull, \
-    0x000003a247fc03cfull, 0x000003a008fc9fc0ull, 0x000003903ffc01f8ull, 0x0000039038fc0fffull, \
-    0x000003a2401c73cfull, 0x00000090071c7fc0ull, 0x00000010381c71f8ull, 0x000003903f1c7fffull, \
-    0x00000391ff03ffffull, 0x000003a3f827f1f8ull, 0x000003a3c727ffc0ull, 0x000003a3c027f1c7ull, \
-    0x00000391f8e38fffull, 0x00000391ffe381f8ull, 0x000003a3c1e79fc0ull, 0x00000391c7e381c7ull, \
-    0x000003a3c7240fffull, 0x000003b5c84891f8ull, 0x000003a3ff240fc0ull, 0x000003a3f82401c7ull, \
-    0x00000391c0e07fffull, 0x00000391c7e071f8ull, 0x00000391f8e07fc0ull, 0x00000391ffe071c7ull, \
-    0x0000039fff03f03full, 0x0000039ff803fe38ull, 0x000003afc727f240ull, 0x0000039fc003fe07ull, \
-    0x0000039ff8e3803full, 0x0000039fffe38e38ull, 0x000003afc1e79240ull, 0x0000039fc7e38e07ull, \
-    0x000003afc724027full, 0x000003afc8009e38ull, 0x000003afff240240ull, 0x0000019ff8000e07ull, \
-    0x0000039fc0e0703full, 0x0000039fc7e07e38ull, 0x0000011ff8e07000ull, 0x0000039fffe07e07ull, \
-    0x000003a23f27fe07ull, 0x000003b4784bf048ull, 0x000003a20727fe38ull, 0x000003a20027f03full, \
-    0x000003a078e38e4full, 0x000003a07fe38048ull, 0x000003a201e79e38ull, 0x0000029007e3803full, \
-    0x000003b447480e4full, 0x000003c6886c9090ull, 0x000003a23f240e38ull, 0x000003a23824003full, \
-    0x000003a040e07e4full, 0x000003a047e07048ull, 0x0000029038e07e38ull, 0x000002903fe0703full, \
-    0x0000039e3f03f1c7ull, 0x0000039e3803ffc0ull, 0x0000039e0703f1f8ull, 0x0000039e0003ffffull, \
-    0x0000039e38e381c7ull, 0x0000039e3fe38fc0ull, 0x0000021e00e381f8ull, 0x0000039e07e38fffull, \
-    0x000003ae472403cfull, 0x000003ae08009fc0ull, 0x0000019e3f0001f8ull, 0x0000019e38000fffull, \
-    0x0000009e00e071c7ull, 0x0000009e07e07fc0ull, 0x000003ae78e4f1f8ull, 0x0000039e3fe07fffull, \
-    0x0000039e3fe07fffull, 0x000003ae78e4f1f8ull, 0x000003ae47e4ffc0ull, 0x000003ae40e4f1c7ull, \
-    0x000003ae78048fffull, 0x000003ae7f0481f8ull, 0x000003be81291fc0ull, 0x000003ae470481c7ull, \
-    0x0000039e07e38fffull, 0x000003ae48e791f8ull, 0x0000039e3fe38fc0ull, 0x0000039e38e381c7ull, \
-    0x0000039e0003ffffull, 0x0000039e0703f1f8ull, 0x0000039e3803ffc0ull, 0x0000039e3f03f1c7ull, \
-    0x000003a07fe4f03full, 0x000003a078e4fe38ull, 0x000003b287e97240ull, 0x000003a040e4fe07ull, \
-    0x000003b2b809027full, 0x000003a07f048e38ull, 0x000003c4c12d9480ull, 0x000003a047048e07ull, \
-    0x000003a207e3827full, 0x000003a048e79e38ull, 0x000003a23fe38240ull, 0x0000029038e38e07ull, \
-    0x000003a20003f27full, 0x000002900703fe38ull, 0x000003a23803f240ull, 0x000002903f03fe07ull, \
-    0x0000039fffe07e07ull, 0x000003aff8e4f048ull, 0x0000039fc7e07e38ull, 0x0000039fc0e0703full, \
-    0x000003aff8048e4full, 0x000003afff048048ull, 0x000003afc1201e38ull, 0x0000019fc700003full, \
-    0x0000039fc7e38e07ull, 0x000003afc8e79048ull, 0x0000039fffe38e38ull, 0x0000039ff8e3803full, \
-    0x0000039fc003fe07ull, 0x0000011fc703f000ull, 0x0000039ff803fe38ull, 0x0000039fff03f03full, \
-    0x00000391ffe071c7ull, 0x00000391f8e07fc0ull, 0x00000391c7e071f8ull, 0x00000391c0e07fffull, \
-    0x000003a3f80483cfull, 0x00000191ff000fc0ull, 0x000003a1c12011f8ull, 0x00000191c7000fffull, \
-    0x00000391c7e381c7ull, 0x00000211c0e38fc0ull, 0x00000391ffe381f8ull, 0x00000391f8e38fffull, \
-    0x00000011c003f1c7ull, 0x000003a3c727ffc0ull, 0x00000011f803f1f8ull, 0x00000391ff03ffffull, \
-    0x000003903f1c7fffull, 0x000003a2783cf1f8ull, 0x000003a2473cffc0ull, 0x000003a2403cf1c7ull, \
-    0x0000039038fc0fffull, 0x000003903ffc01f8ull, 0x000003a241fc9fc0ull, 0x0000009007fc01c7ull, \
-    0x00000390071f8fffull, 0x000003a2483f91f8ull, 0x000003903f1f8fc0ull, 0x00000010381f81c7ull, \
-    0x0000039000ffffffull, 0x0000039007fff1f8ull, 0x0000039038ffffc0ull, 0x000003903ffff1c7ull, \
-    0x0000039e3f1c703full, 0x0000039e381c7e38ull, 0x000003ae473cf240ull, 0x0000009e001c7e07ull, \
-    0x0000039e38fc003full, 0x0000039e3ffc0e38ull, 0x000003ae41fc9240ull, 0x0000009e07fc0e07ull, \
-    0x0000039e071f803full, 0x0000021e001f8e38ull, 0x0000011e3f1f8000ull, 0x000003ae781f8e4full, \
-    0x0000039e00fff03full, 0x0000039e07fffe38ull, 0x0000011e38fff000ull, 0x0000039e3ffffe07ull, \
-    0x00000391ff1c7e07ull, 0x000003a3f83cf048ull, 0x00000391c71c7e38ull, 0x00000011c01c703full, \
-    0x00000391f8fc0e07ull, 0x00000111fffc0000ull, 0x00000211c0fc0e38ull, 0x000003a3c7fc027full, \
-    0x00000391c71f8e07ull, 0x000003a3c83f9048ull, 0x00000391ff1f8e38ull, 0x00000011f81f803full, \
-    0x00000391c0fffe07ull, 0x00000111c7fff000ull, 0x00000391f8fffe38ull, 0x00000391fffff03full, \
-    0x0000039fff1c71c7ull, 0x0000039ff81c7fc0ull, 0x0000039fc71c71f8ull, 0x0000039fc01c7fffull, \
-    0x0000039ff8fc01c7ull, 0x0000039ffffc0fc0ull, 0x0000021fc0fc01f8ull, 0x0000039fc7fc0fffull, \
-    0x0000039fc71f81c7ull, 0x0000021fc01f8fc0ull, 0x0000039fff1f81f8ull, 0x0000039ff81f8fffull, \
-    0x0000039fc0fff1c7ull, 0x0000039fc7ffffc0ull, 0x0000039ff8fff1f8ull, 0x0000038fffffffffull,
+    0x0000038fffffffffull, 0x00000f9ff8fff1f8ull, 0x00000f9fc7ffffc0ull, 0x0000139fc0fff1c7ull, \
+    0x00000f9ff81f8fffull, 0x0000139fff1f81f8ull, 0x00006fafc13f9fc0ull, 0x0000179fc71f81c7ull, \
+    0x00000f9fc7fc0fffull, 0x00006fafc8fc91f8ull, 0x0000139ffffc0fc0ull, 0x0000179ff8fc01c7ull, \
+    0x0000139fc01c7fffull, 0x0000179fc71c71f8ull, 0x0000179ff81c7fc0ull, 0x0000139fff1c71c7ull, \
+    0x00000f91fffff03full, 0x00001391f8fffe38ull, 0x00006fa3c7fff240ull, 0x00001791c0fffe07ull, \
+    0x00006fa3f81f827full, 0x00001791ff1f8e38ull, 0x00036fb5c13f9480ull, 0x00001b91c71f8e07ull, \
+    0x00006fa3c7fc027full, 0x000073a1c8fc9e38ull, 0x000073a3fffc0240ull, 0x00001b91f8fc0e07ull, \
+    0x000073a3c01c727full, 0x00001b91c71c7e38ull, 0x000077a3f81c7240ull, 0x00001791ff1c7e07ull, \
+    0x00000f9e3ffffe07ull, 0x00006fae78fff048ull, 0x0000139e07fffe38ull, 0x0000179e00fff03full, \
+    0x00006fae781f8e4full, 0x000073ae7f1f8048ull, 0x000073ae013f9e38ull, 0x00001b9e071f803full, \
+    0x00006fae47fc0e4full, 0x00036fbe88fc9090ull, 0x0000179e3ffc0e38ull, 0x00001b9e38fc003full, \
+    0x000073ae401c7e4full, 0x000077ae471c7048ull, 0x00001b9e381c7e38ull, 0x0000179e3f1c703full, \
+    0x000013903ffff1c7ull, 0x0000179038ffffc0ull, 0x0000179007fff1f8ull, 0x0000139000ffffffull, \
+    0x00008fa2781f83cfull, 0x00001b903f1f8fc0ull, 0x000077a0013f91f8ull, 0x00001790071f8fffull, \
+    0x00008fa247fc03cfull, 0x000077a008fc9fc0ull, 0x00001b903ffc01f8ull, 0x0000179038fc0fffull, \
+    0x000093a2401c73cfull, 0x00001c90071c7fc0ull, 0x00001c10381c71f8ull, 0x000013903f1c7fffull, \
+    0x00000f91ff03ffffull, 0x00006fa3f827f1f8ull, 0x00006fa3c727ffc0ull, 0x000073a3c027f1c7ull, \
+    0x00001391f8e38fffull, 0x00001791ffe381f8ull, 0x00008fa3c1e79fc0ull, 0x00001b91c7e381c7ull, \
+    0x00006fa3c7240fffull, 0x00036fb5c84891f8ull, 0x000073a3ff240fc0ull, 0x000077a3f82401c7ull, \
+    0x00001791c0e07fffull, 0x00001b91c7e071f8ull, 0x00001b91f8e07fc0ull, 0x00001791ffe071c7ull, \
+    0x0000139fff03f03full, 0x0000179ff803fe38ull, 0x00008fafc727f240ull, 0x00001b9fc003fe07ull, \
+    0x0000179ff8e3803full, 0x0000139fffe38e38ull, 0x0000afafc1e79240ull, 0x0000179fc7e38e07ull, \
+    0x00008fafc724027full, 0x000077afc8009e38ull, 0x000093afff240240ull, 0x00001d9ff8000e07ull, \
+    0x00001b9fc0e0703full, 0x0000179fc7e07e38ull, 0x00001d1ff8e07000ull, 0x0000139fffe07e07ull, \
+    0x00006fa23f27fe07ull, 0x00036fb4784bf048ull, 0x000073a20727fe38ull, 0x000077a20027f03full, \
+    0x000073a078e38e4full, 0x000077a07fe38048ull, 0x000093a201e79e38ull, 0x00001e9007e3803full, \
+    0x00036fb447480e4full, 0x001b6fc6886c9090ull, 0x000077a23f240e38ull, 0x00007ba23824003full, \
+    0x000077a040e07e4full, 0x00007ba047e07048ull, 0x00001e9038e07e38ull, 0x00001a903fe0703full, \
+    0x0000179e3f03f1c7ull, 0x00001b9e3803ffc0ull, 0x00001b9e0703f1f8ull, 0x0000179e0003ffffull, \
+    0x00001b9e38e381c7ull, 0x0000179e3fe38fc0ull, 0x00001e1e00e381f8ull, 0x0000139e07e38fffull, \
+    0x0000afae472403cfull, 0x00007bae08009fc0ull, 0x00001d9e3f0001f8ull, 0x0000199e38000fffull, \
+    0x00001c9e00e071c7ull, 0x0000189e07e07fc0ull, 0x00006fae78e4f1f8ull, 0x00000f9e3fe07fffull, \
+    0x00000f9e3fe07fffull, 0x00006fae78e4f1f8ull, 0x00006fae47e4ffc0ull, 0x000073ae40e4f1c7ull, \
+    0x00006fae78048fffull, 0x000073ae7f0481f8ull, 0x00036fbe81291fc0ull, 0x000077ae470481c7ull, \
+    0x0000139e07e38fffull, 0x00008fae48e791f8ull, 0x0000179e3fe38fc0ull, 0x00001b9e38e381c7ull, \
+    0x0000179e0003ffffull, 0x00001b9e0703f1f8ull, 0x00001b9e3803ffc0ull, 0x0000179e3f03f1c7ull, \
+    0x00006fa07fe4f03full, 0x000073a078e4fe38ull, 0x00036fb287e97240ull, 0x000077a040e4fe07ull, \
+    0x00036fb2b809027full, 0x000077a07f048e38ull, 0x001b6fc4c12d9480ull, 0x00007ba047048e07ull, \
+    0x000073a207e3827full, 0x000093a048e79e38ull, 0x000077a23fe38240ull, 0x00001e9038e38e07ull, \
+    0x000077a20003f27full, 0x00001e900703fe38ull, 0x00007ba23803f240ull, 0x00001a903f03fe07ull, \
+    0x0000139fffe07e07ull, 0x00008faff8e4f048ull, 0x0000179fc7e07e38ull, 0x00001b9fc0e0703full, \
+    0x00008faff8048e4full, 0x000093afff048048ull, 0x000077afc1201e38ull, 0x00001d9fc700003full, \
+    0x0000179fc7e38e07ull, 0x0000afafc8e79048ull, 0x0000139fffe38e38ull, 0x0000179ff8e3803full, \
+    0x00001b9fc003fe07ull, 0x00001d1fc703f000ull, 0x0000179ff803fe38ull, 0x0000139fff03f03full, \
+    0x00001791ffe071c7ull, 0x00001b91f8e07fc0ull, 0x00001b91c7e071f8ull, 0x00001791c0e07fffull, \
+    0x0000afa3f80483cfull, 0x00001d91ff000fc0ull, 0x00007ba1c12011f8ull, 0x00001991c7000fffull, \
+    0x00001b91c7e381c7ull, 0x00001e11c0e38fc0ull, 0x00001791ffe381f8ull, 0x00001391f8e38fffull, \
+    0x00001c11c003f1c7ull, 0x00006fa3c727ffc0ull, 0x00001811f803f1f8ull, 0x00000f91ff03ffffull, \
+    0x000013903f1c7fffull, 0x00008fa2783cf1f8ull, 0x00008fa2473cffc0ull, 0x000093a2403cf1c7ull, \
+    0x0000179038fc0fffull, 0x00001b903ffc01f8ull, 0x0000afa241fc9fc0ull, 0x00001c9007fc01c7ull, \
+    0x00001790071f8fffull, 0x0000afa2483f91f8ull, 0x00001b903f1f8fc0ull, 0x00001c10381f81c7ull, \
+    0x0000139000ffffffull, 0x0000179007fff1f8ull, 0x0000179038ffffc0ull, 0x000013903ffff1c7ull, \
+    0x0000179e3f1c703full, 0x00001b9e381c7e38ull, 0x0000afae473cf240ull, 0x00001c9e001c7e07ull, \
+    0x00001b9e38fc003full, 0x0000179e3ffc0e38ull, 0x0000cfae41fc9240ull, 0x0000189e07fc0e07ull, \
+    0x00001b9e071f803full, 0x00001e1e001f8e38ull, 0x00001d1e3f1f8000ull, 0x00006fae781f8e4full, \
+    0x0000179e00fff03full, 0x0000139e07fffe38ull, 0x0000191e38fff000ull, 0x00000f9e3ffffe07ull, \
+    0x00001791ff1c7e07ull, 0x0000afa3f83cf048ull, 0x00001b91c71c7e38ull, 0x00001c11c01c703full, \
+    0x00001b91f8fc0e07ull, 0x00001d11fffc0000ull, 0x00001e11c0fc0e38ull, 0x00006fa3c7fc027full, \
+    0x00001b91c71f8e07ull, 0x0000cfa3c83f9048ull, 0x00001791ff1f8e38ull, 0x00001811f81f803full, \
+    0x00001791c0fffe07ull, 0x00001911c7fff000ull, 0x00001391f8fffe38ull, 0x00000f91fffff03full, \
+    0x0000139fff1c71c7ull, 0x0000179ff81c7fc0ull, 0x0000179fc71c71f8ull, 0x0000139fc01c7fffull, \
+    0x0000179ff8fc01c7ull, 0x0000139ffffc0fc0ull, 0x00001a1fc0fc01f8ull, 0x00000f9fc7fc0fffull, \
+    0x0000179fc71f81c7ull, 0x00001a1fc01f8fc0ull, 0x0000139fff1f81f8ull, 0x00000f9ff81f8fffull, \
+    0x0000139fc0fff1c7ull, 0x00000f9fc7ffffc0ull, 0x00000f9ff8fff1f8ull, 0x0000038fffffffffull,
 static const uint64_t DISO_DMC_CASE[256] = {DISO_DMC_CASE_VALUES};
 
 // DMC quad stencil (src/cudualmc.cu:504-513): for a crossing edge owned by a cell and of type
@@ -189,5 +190,6 @@ static const int DISO_QUAD[6][4][4] = {DISO_QUAD_VALUES};
 #define DISO_DMC_PATCH_OF(w, e) ((int)(((w) >> (3 * (e))) & 0x7ull))
 #define DISO_DMC_NPATCH(w) ((int)(((w) >> 36) & 0x7ull))
 #define DISO_DMC_DIR(w) ((int)(((w) >> 39) & 0x7ull))
+#define DISO_DMC_PATCH_SIZE(w, k) ((int)(((w) >> (42 + 3 * (k))) & 0x7ull))
 
 #endif

@@ -17,18 +17,23 @@ _DTYPES = {torch.float32: _lib.F32, torch.float64: _lib.F64}
 
 
 class Counted:
-    """Result of the counting pass: segment bases on the device and the totals on the host."""
+    """Result of the counting pass: segment bases on the device and the totals on the host.
+    After emission it also holds what a DMC backward needs from the forward: the quads and one
+    byte of patch information per DMC vertex."""
 
-    __slots__ = ("seg", "n_verts", "n_faces", "flags", "dims", "iso", "flags_in")
+    __slots__ = ("seg", "n_verts", "n_faces", "n_crossing", "flags", "dims", "iso", "flags_in", "quads", "vinfo")
 
     def __init__(self, seg, totals, dims, iso, flags_in):
         self.seg = seg
         self.n_verts = int(totals.n_verts)
         self.n_faces = int(totals.n_faces)
+        self.n_crossing = int(totals.n_crossing)
         self.flags = int(totals.flags)
         self.dims = dims
         self.iso = iso
         self.flags_in = flags_in
+        self.quads = None
+        self.vinfo = None
 
     @property
     def empty_grid(self):
@@ -94,17 +99,28 @@ class Extractor:
         _lib.check(rc, "count pass")
         return Counted(seg, tot, dims, float(iso), flags)
 
+    def _scratch(self, counted, device):
+        n = _lib.lib().diso_scratch_bytes(self.algo, self.code, counted.n_crossing)
+        if n < 0:
+            _lib.check(n, "scratch sizing")
+        return torch.empty(max(n, 1), dtype=torch.uint8, device=device), n
+
     def emit(self, grid, deform, counted, flags):
         """Vertices [V,3] and faces ([F,3] MC triangles / [Q,4] DMC quads) of a counted grid."""
         L = _lib.lib()
-        k = 3 if self.algo == _lib.MC else 4
+        dmc = self.algo == _lib.DMC
+        k = 4 if dmc else 3
         idx = torch.int64 if flags & _lib.FLAG_INDEX64 else torch.int32
         verts = torch.empty((counted.n_verts, 3), dtype=self.dtype, device=grid.device)
         faces = torch.empty((counted.n_faces, k), dtype=idx, device=grid.device)
+        vinfo = torch.empty(counted.n_verts if dmc else 0, dtype=torch.uint8, device=grid.device)
+        scratch, nscratch = self._scratch(counted, grid.device)
         rc = L.diso_emit(self.algo, self.code, _ptr(grid), _ptr(deform), *counted.dims, counted.iso, flags,
                          _ptr(counted.seg), counted.seg.numel(), _ptr(verts), counted.n_verts, _ptr(faces),
-                         counted.n_faces, _stream(grid.device))
+                         counted.n_faces, _ptr(vinfo), _ptr(scratch), nscratch, _stream(grid.device))
         _lib.check(rc, "emit pass")
+        if dmc:
+            counted.quads, counted.vinfo = faces, vinfo
         return verts, faces
 
     def backward(self, grid, deform, counted, flags, adj_verts, adj_grid, adj_deform):
@@ -115,7 +131,16 @@ class Extractor:
             raise RuntimeError("adj_verts type must match the extractor's dtype")
         if adj_verts.shape != (counted.n_verts, 3):
             raise ValueError(f"adj_verts must have shape ({counted.n_verts}, 3), got {tuple(adj_verts.shape)}")
+        dmc = self.algo == _lib.DMC
+        quads = counted.quads if dmc else None
+        if dmc and quads is None:
+            raise RuntimeError("DMC backward needs the quads of its forward")
+        if dmc:
+            # the quads' index width decides how the backward reads them
+            flags = (flags & ~_lib.FLAG_INDEX64) | (_lib.FLAG_INDEX64 if quads.dtype == torch.int64 else 0)
+        scratch, nscratch = self._scratch(counted, grid.device)
         rc = L.diso_backward(self.algo, self.code, _ptr(grid), _ptr(deform), *counted.dims, counted.iso, flags,
-                             _ptr(counted.seg), counted.seg.numel(), _ptr(adj_verts), counted.n_verts,
-                             _ptr(adj_grid), _ptr(adj_deform), _stream(grid.device))
+                             _ptr(counted.seg), counted.seg.numel(), _ptr(quads), counted.n_faces if dmc else 0,
+                             _ptr(counted.vinfo if dmc else None), _ptr(adj_verts), counted.n_verts,
+                             _ptr(adj_grid), _ptr(adj_deform), _ptr(scratch), nscratch, _stream(grid.device))
         _lib.check(rc, "backward pass")

@@ -93,16 +93,23 @@ int diso_count(int algo, int dtype, const void *sdf, int64_t X, int64_t Y, int64
                int flags, void *seg, int64_t seg_bytes, void *ctl, int64_t ctl_bytes,
                diso_totals_t *totals_host, void *stream);
 
+/* Size in bytes of the scratch buffer diso_emit and diso_backward need for a counted grid
+ * (n_crossing from diso_totals_t): none for MC; for DMC, the MC edge vertices (forward) or the
+ * per-edge upstream gradients (backward), n_crossing * 3 scalars. */
+int64_t diso_scratch_bytes(int algo, int dtype, int64_t n_crossing);
+
 /* Pass 2: write vertices [n_verts,3] and faces ([n_faces,3] MC triangles, [n_faces,4] DMC quads)
  * in the reference's order. Replaces create_cell_mc_verts_kernel + create_cell_mc_tris_kernel
  * (src/cumc.cu:370-410,564-649) and create_dmc_verts_kernel + create_quads_kernel
  * (src/cudualmc.cu:907-955,1027-1056), plus the wrapper's shift / normalise / int64 epilogue
  * (diso/__init__.py:56-61,110-116) when the corresponding flags are set. Writes never exceed
- * vert_capacity / face_capacity (rows); the call fails with DISO_E_CAPACITY if the counts from
- * diso_count do not fit (checked on the device; the outputs are then not written). */
+ * vert_capacity / face_capacity (rows); if the counts from diso_count do not fit, nothing is
+ * written. DMC only: `scratch` (diso_scratch_bytes) is overwritten, and `vinfo` receives one
+ * byte per DMC vertex (edges in its patch | patch rank << 3) that diso_backward needs. */
 int diso_emit(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y,
               int64_t Z, double iso, int flags, const void *seg, int64_t seg_bytes, void *verts,
-              int64_t vert_capacity, void *faces, int64_t face_capacity, void *stream);
+              int64_t vert_capacity, void *faces, int64_t face_capacity, void *vinfo,
+              void *scratch, int64_t scratch_bytes, void *stream);
 
 /* Backward: gradients of sum(adj_verts * verts) with respect to sdf [X,Y,Z] and deform
  * [X,Y,Z,3] (adj_deform may be NULL). One thread per grid vertex gathers from its at most six
@@ -112,11 +119,14 @@ int diso_emit(int algo, int dtype, const void *sdf, const void *deform, int64_t 
  * the slicing of F.pad's backward. adj_verts is the gradient with respect to the vertices as
  * emitted with the same flags. Gradients are written for every grid vertex (zero where no
  * crossing edge touches it) unless DISO_FLAG_ACCUMULATE is set. `seg` must hold the result of a
- * diso_count on the same inputs. */
+ * diso_count on the same inputs. DMC only: `faces` / `vinfo` are the quads (in the index width
+ * of `flags`) and vertex info written by diso_emit with the same inputs, n_faces the quad count,
+ * and `scratch` (diso_scratch_bytes) is overwritten. */
 int diso_backward(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y,
                   int64_t Z, double iso, int flags, const void *seg, int64_t seg_bytes,
-                  const void *adj_verts, int64_t n_verts, void *adj_sdf, void *adj_deform,
-                  void *stream);
+                  const void *faces, int64_t n_faces, const void *vinfo, const void *adj_verts,
+                  int64_t n_verts, void *adj_sdf, void *adj_deform, void *scratch,
+                  int64_t scratch_bytes, void *stream);
 
 /* Reads the totals of the last diso_count on `seg` (blocking copy on `stream`). */
 int diso_read_totals(const void *seg, int64_t seg_bytes, int64_t X, int64_t Y, int64_t Z,

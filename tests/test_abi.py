@@ -55,10 +55,17 @@ def test_invalid_arguments_fail_before_any_device_work():
     fake = ctypes.c_void_p(0x1000)
     rc = L.diso_count(0, 0, fake, 4, 4, 4, 0.0, 0, fake, 16, fake, 1 << 20, ctypes.byref(tot), null)
     assert rc == -3 and b"seg" in L.diso_last_error()
-    rc = L.diso_emit(0, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 1 << 20, null, 5, null, 0, null)
+    rc = L.diso_emit(0, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 1 << 20, null, 5, null, 0, null, null, 0, null)
     assert rc == -1
-    rc = L.diso_backward(0, 0, fake, fake, 4, 4, 4, 0.0, 0, fake, 1 << 20, fake, 3, fake, null, null)
+    rc = L.diso_emit(1, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 1 << 20, fake, 5, fake, 7, fake, fake, 16, null)
+    assert rc == -3 and b"scratch" in L.diso_last_error()
+    rc = L.diso_backward(0, 0, fake, fake, 4, 4, 4, 0.0, 0, fake, 1 << 20, null, 0, null, fake, 3, fake, null,
+                         null, 0, null)
     assert rc == -1 and b"adj_deform" in L.diso_last_error()
+    rc = L.diso_backward(1, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 1 << 20, null, 9, null, fake, 3, fake, null,
+                         null, 0, null)
+    assert rc == -1 and b"quads" in L.diso_last_error()
+    assert L.diso_scratch_bytes(0, 0, 100) == 0 and L.diso_scratch_bytes(1, 1, 100) == 2400
     with pytest.raises(RuntimeError, match="count pass failed"):
         _lib.check(-1, "count pass")
 
