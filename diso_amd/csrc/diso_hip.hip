@@ -233,6 +233,144 @@ __device__ __forceinline__ int64_t seg_index(const Geo &g, int px, int py, int c
 }
 
 // ------------------------------------------------------------------------------------------
+// Register windows and edge geometry shared by the emission and backward kernels
+
+// Samples of a lane's neighbourhood held in registers: v[i][j][k] is the sample at
+// (px + X0 + i, py + Y0 + j, pz + Z0 + k). Only ever indexed with compile-time constants. Boxes
+// of it are loaded on demand (load_box), so a kernel can test cheaply for work before loading
+// the rest. When the whole window of the whole wave lies inside the stored grid (`interior`,
+// wave-uniform: px, py and the chunk are), samples are read through a row pointer without
+// per-sample range checks.
+template <typename S, int X0, int NX, int Y0, int NY, int Z0, int NZ>
+struct Win {
+    S v[NX][NY][NZ];
+    int px, py, pz;
+    bool interior;
+    __device__ __forceinline__ Win(const In<S> &in, int px_, int py_, int pz_, int z0) : px(px_), py(py_), pz(pz_) {
+        const Geo &g = in.geo;
+        interior = px + X0 - g.off >= 0 && px + X0 + NX - 1 - g.off < g.n0 && py + Y0 - g.off >= 0 &&
+                   py + Y0 + NY - 1 - g.off < g.n1 && z0 + Z0 - g.off >= 0 &&
+                   z0 + kWave - 1 + Z0 + NZ - 1 - g.off < g.n2;
+    }
+    template <int BX0, int BNX, int BY0, int BNY, int BZ0, int BNZ>
+    __device__ __forceinline__ void load_box(const In<S> &in) {
+        static_assert(BX0 >= X0 && BX0 + BNX <= X0 + NX && BY0 >= Y0 && BY0 + BNY <= Y0 + NY && BZ0 >= Z0 &&
+                          BZ0 + BNZ <= Z0 + NZ,
+                      "box outside the window");
+        const Geo &g = in.geo;
+        const int64_t sy = g.n2, sx = (int64_t)g.n1 * g.n2;
+        if (interior) {
+            const S *p = in.sdf + ((int64_t)(px + BX0 - g.off) * g.n1 + (py + BY0 - g.off)) * g.n2 + (pz + BZ0 - g.off);
+#pragma unroll
+            for (int i = 0; i < BNX; ++i)
+#pragma unroll
+                for (int j = 0; j < BNY; ++j)
+#pragma unroll
+                    for (int k = 0; k < BNZ; ++k) v[BX0 - X0 + i][BY0 - Y0 + j][BZ0 - Z0 + k] = p[i * sx + j * sy + k];
+        } else {
+#pragma unroll
+            for (int i = 0; i < BNX; ++i)
+#pragma unroll
+                for (int j = 0; j < BNY; ++j)
+#pragma unroll
+                    for (int k = 0; k < BNZ; ++k)
+                        v[BX0 - X0 + i][BY0 - Y0 + j][BZ0 - Z0 + k] = sample(in, px + BX0 + i, py + BY0 + j, pz + BZ0 + k);
+        }
+    }
+    __device__ __forceinline__ S operator()(int dx, int dy, int dz) const { return v[dx - X0][dy - Y0][dz - Z0]; }
+};
+
+// Segment of a wave: its row (px, py) and chunk, from the wave-uniform segment id.
+struct SegPos {
+    int px, py, chunk;
+};
+__device__ __forceinline__ SegPos seg_pos(const Geo &g, int64_t s) {
+    const uint32_t su = (uint32_t)s, nc = (uint32_t)g.nchunk, g1 = (uint32_t)g.g1;
+    const uint32_t row = su / nc;
+    return SegPos{(int)(row / g1), (int)(row % g1), (int)(su % nc)};
+}
+
+// Wave index within the block, as a scalar (wave-uniform) value.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); }
+
+// Edge e of a cell (src/cumc.cu:109-122): starts at cell-relative corner (eox, eoy, eoz) and
+// runs along axis eax. Bit masks of the edge table, so unrolled loops fold them to constants.
+__device__ constexpr int eox(int e) { return (0x622 >> e) & 1; }
+__device__ constexpr int eoy(int e) { return (0x0F0 >> e) & 1; }
+__device__ constexpr int eoz(int e) { return (0xC44 >> e) & 1; }
+__device__ constexpr int eax(int e) { return e >= 8 ? 1 : ((e & 1) ? 2 : 0); }
+
+// Crossing mask of the three edges a cell owns, from its sample and its +x, +y, +z neighbours.
+template <typename S>
+__device__ __forceinline__ int own_mask(S d0, S dx, S dy, S dz, S iso) {
+    return crosses(d0, dx, iso) | (crosses(d0, dy, iso) << 1) | (crosses(d0, dz, iso) << 2);
+}
+
+// DMC code of the cell anchored at window offset (cx,cy,cz) (corners cx..cx+1 etc.), resolved
+// with the face-neighbour rule; 0 when the position is not a cell.
+template <typename S, typename W>
+__device__ __forceinline__ int dmc_code_w(const In<S> &in, const uint64_t *cases, const W &w, int px, int py,
+                                          int pz, int cx, int cy, int cz) {
+    if (!is_anchor(in, px + cx, py + cy, pz + cz)) return 0;
+    int code = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) code |= (w(cx + (i & 1), cy + ((i >> 1) & 1), cz + ((i >> 2) & 1)) >= in.iso) << i;
+    if (code == 0 || code == 255) return code;
+    return dmc_good_code(in, cases, code, px + cx, py + cy, pz + cz);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename S>
+__device__ __forceinline__ V3<S> add3(V3<S> a, V3<S> b) {
+    return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+
+// computeMcVert from samples and deforms already in registers: p0 + (p1 - p0) * t with
+// p = integer position + deform (src/cumc.cu:343-368).
+template <typename S, bool DEF>
+__device__ __forceinline__ V3<S> lerp_vertex(int x, int y, int z, int axis, S d0, S d1, V3<S> f0, V3<S> f1,
+                                             S iso) {
+    const S t = interp_t(d0, d1, iso);
+    S p0x = S(x), p0y = S(y), p0z = S(z);
+    S p1x = S(x + (axis == 0)), p1y = S(y + (axis == 1)), p1z = S(z + (axis == 2));
+    if (DEF) {
+        p0x += f0.x;
+        p0y += f0.y;
+        p0z += f0.z;
+        p1x += f1.x;
+        p1y += f1.y;
+        p1z += f1.z;
+    }
+    V3<S> r;
+    r.x = p0x + (p1x - p0x) * t;
+    r.y = p0y + (p1y - p0y) * t;
+    r.z = p0z + (p1z - p0z) * t;
+    return r;
+}
+
+// x / c for a divisor c fixed per call, with r = RN(1/c): q = x*r is within an ulp of x/c and
+// one FMA correction with the exact residual makes it the correctly rounded quotient
+// (Markstein's theorem), i.e. bit-identical to IEEE x / c, in 3 instructions instead of the
+// ~10 of a general correctly rounded division.
+template <typename S>
+__device__ __forceinline__ S div_by(S x, S c, S r) {
+    if (c == S(0)) return x / c;  // uniform; keeps IEEE inf/nan for a length-1 axis
+    const S q = x * r;
+    return fma(fma(-q, c, x), r, q);
+}
+
+// Per-axis divisors (dims - 1) and their reciprocals, for the wrapper's normalisation.
+template <typename S>
+struct Norm {
+    S c[3], r[3];
+};
+
+// ------------------------------------------------------------------------------------------
 // Pass 1: classify, count, scan (single pass)
 //
 // The grid of segments is split into G contiguous ranges, one per block (G ~ 4 blocks per CU, so
@@ -272,28 +410,35 @@ __device__ __forceinline__ uint64_t pack_agg(int64_t v, int64_t f, uint32_t fl) 
 // Per-position counts for pass 1: nv = crossing edges owned (MC vertices / DMC quads);
 // nf = MC triangles / DMC patches; fl = sample flags of the position's own sample.
 template <typename S, int ALGO>
-__device__ __forceinline__ void count_position(const In<S> &in, const uint64_t *cases, int px, int py,
-                                               int pz, int *nv, int *nf, uint32_t *fl) {
+__device__ __forceinline__ void count_position(const In<S> &in, const uint64_t *cases, int px, int py, int pz,
+                                               int z0, int *nv, int *nf, uint32_t *fl) {
+    Win<S, 0, 2, 0, 2, 0, 2> w(in, px, py, pz, z0);
+    w.template load_box<0, 2, 0, 2, 0, 2>(in);
+    const S v = w(0, 0, 0);
+    *fl = in_data(in, px, py, pz) ? ((v < in.iso ? DISO_TOTAL_HAS_BELOW : 0u) |
+                                     (v > in.iso ? DISO_TOTAL_HAS_ABOVE : 0u) | (v != v ? DISO_TOTAL_HAS_NAN : 0u))
+                                  : 0u;
     *nv = 0;
     *nf = 0;
-    *fl = 0;
-    if (in_data(in, px, py, pz)) {
-        S v = in.sdf[data_index(in, px, py, pz)];
-        *fl = (v < in.iso ? DISO_TOTAL_HAS_BELOW : 0u) | (v > in.iso ? DISO_TOTAL_HAS_ABOVE : 0u) |
-              (v != v ? DISO_TOTAL_HAS_NAN : 0u);
-    }
     if (!is_anchor(in, px, py, pz)) return;
-    Corners<S> k = load_corners(in, px, py, pz);
-    S d0 = k.c[0][0][0];
-    *nv = crosses(d0, k.c[1][0][0], in.iso) + crosses(d0, k.c[0][1][0], in.iso) +
-          crosses(d0, k.c[0][0][1], in.iso);
+    *nv = crosses(v, w(1, 0, 0), in.iso) + crosses(v, w(0, 1, 0), in.iso) + crosses(v, w(0, 0, 1), in.iso);
     if (ALGO == DISO_MC) {
+        Corners<S> k;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) k.c[i & 1][(i >> 1) & 1][(i >> 2) & 1] = w(i & 1, (i >> 1) & 1, (i >> 2) & 1);
         *nf = DISO_MC_NTRI(cases[mc_code(k, in.iso)]);
     } else {
-        int code = dmc_code(k, in.iso);
+        int code = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) code |= (w(i & 1, (i >> 1) & 1, (i >> 2) & 1) >= in.iso) << i;
         if (code != 0 && code != 255) code = dmc_good_code(in, cases, code, px, py, pz);
         *nf = DISO_DMC_NPATCH(cases[code]);
     }
+}
+
+// Wave sum of a per-lane value in [0, 7] (three ballots).
+__device__ __forceinline__ int wave_total_small(int v) {
+    return __popcll(__ballot(v & 1)) + 2 * __popcll(__ballot(v & 2)) + 4 * __popcll(__ballot(v & 4));
 }
 
 __device__ __forceinline__ int64_t wave_incl64(int64_t v) {
@@ -316,7 +461,7 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *s
 
     CtlHeader *hdr = reinterpret_cast<CtlHeader *>(ctl);
     uint64_t *words = reinterpret_cast<uint64_t *>(ctl + kCtlHeader);
-    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     const Geo &g = in.geo;
     const int64_t b = blockIdx.x;
     const int64_t lo = b * g.nseg / nblocks, hi = (b + 1) * g.nseg / nblocks;
@@ -335,15 +480,11 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *s
     int64_t wv = 0, wf = 0;
     uint32_t wfl = 0;
     for (int64_t s = lo + wave; s < hi; s += kCountWaves) {  // wave-uniform
-        const int chunk = (int)(s % g.nchunk);
-        const int64_t row = s / g.nchunk;
+        const SegPos sp = seg_pos(g, s);
         int nv, nf;
         uint32_t fl;
-        count_position<S, ALGO>(in, s_case, (int)(row / g.g1), (int)(row % g.g1), chunk * kWave + lane, &nv,
-                                &nf, &fl);
-        int tv, tf;
-        (void)wave_excl_small(nv, &tv);
-        (void)wave_excl_small(nf, &tf);
+        count_position<S, ALGO>(in, s_case, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave, &nv, &nf, &fl);
+        const int tv = wave_total_small(nv), tf = wave_total_small(nf);
         if (lane == 0) {
             seg[s].v = tv;
             seg[s].f = tf;
@@ -462,113 +603,6 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *s
 }
 
 // ------------------------------------------------------------------------------------------
-// Register windows and edge geometry shared by the emission and backward kernels
-
-// Samples of a lane's neighbourhood held in registers: v[i][j][k] is the sample at
-// (px + X0 + i, py + Y0 + j, pz + Z0 + k). Only ever indexed with compile-time constants.
-template <typename S, int X0, int NX, int Y0, int NY, int Z0, int NZ>
-struct Win {
-    S v[NX][NY][NZ];
-    __device__ __forceinline__ void load(const In<S> &in, int px, int py, int pz) {
-        bool okx[NX], oky[NY], okz[NZ];
-#pragma unroll
-        for (int i = 0; i < NX; ++i) okx[i] = (unsigned)(px + X0 + i - in.geo.off) < (unsigned)in.geo.n0;
-#pragma unroll
-        for (int j = 0; j < NY; ++j) oky[j] = (unsigned)(py + Y0 + j - in.geo.off) < (unsigned)in.geo.n1;
-#pragma unroll
-        for (int k = 0; k < NZ; ++k) okz[k] = (unsigned)(pz + Z0 + k - in.geo.off) < (unsigned)in.geo.n2;
-        const int64_t sy = in.geo.n2, sx = (int64_t)in.geo.n1 * in.geo.n2;
-        const int64_t base = (int64_t)(px + X0 - in.geo.off) * sx + (int64_t)(py + Y0 - in.geo.off) * sy +
-                             (pz + Z0 - in.geo.off);
-#pragma unroll
-        for (int i = 0; i < NX; ++i)
-#pragma unroll
-            for (int j = 0; j < NY; ++j)
-#pragma unroll
-                for (int k = 0; k < NZ; ++k)
-                    v[i][j][k] = (okx[i] && oky[j] && okz[k]) ? in.sdf[base + i * sx + j * sy + k] : in.pad;
-    }
-    __device__ __forceinline__ S operator()(int dx, int dy, int dz) const { return v[dx - X0][dy - Y0][dz - Z0]; }
-};
-
-// Edge e of a cell (src/cumc.cu:109-122): starts at cell-relative corner (eox, eoy, eoz) and
-// runs along axis eax. Bit masks of the edge table, so unrolled loops fold them to constants.
-__device__ constexpr int eox(int e) { return (0x622 >> e) & 1; }
-__device__ constexpr int eoy(int e) { return (0x0F0 >> e) & 1; }
-__device__ constexpr int eoz(int e) { return (0xC44 >> e) & 1; }
-__device__ constexpr int eax(int e) { return e >= 8 ? 1 : ((e & 1) ? 2 : 0); }
-
-// Crossing mask of the three edges a cell owns, from its sample and its +x, +y, +z neighbours.
-template <typename S>
-__device__ __forceinline__ int own_mask(S d0, S dx, S dy, S dz, S iso) {
-    return crosses(d0, dx, iso) | (crosses(d0, dy, iso) << 1) | (crosses(d0, dz, iso) << 2);
-}
-
-// DMC code of the cell anchored at window offset (cx,cy,cz) (corners cx..cx+1 etc.), resolved
-// with the face-neighbour rule; 0 when the position is not a cell.
-template <typename S, typename W>
-__device__ __forceinline__ int dmc_code_w(const In<S> &in, const uint64_t *cases, const W &w, int px, int py,
-                                          int pz, int cx, int cy, int cz) {
-    if (!is_anchor(in, px + cx, py + cy, pz + cz)) return 0;
-    int code = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) code |= (w(cx + (i & 1), cy + ((i >> 1) & 1), cz + ((i >> 2) & 1)) >= in.iso) << i;
-    if (code == 0 || code == 255) return code;
-    return dmc_good_code(in, cases, code, px + cx, py + cy, pz + cz);
-}
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <typename S>
-__device__ __forceinline__ V3<S> add3(V3<S> a, V3<S> b) {
-    return {a.x + b.x, a.y + b.y, a.z + b.z};
-}
-
-// computeMcVert from samples and deforms already in registers: p0 + (p1 - p0) * t with
-// p = integer position + deform (src/cumc.cu:343-368).
-template <typename S, bool DEF>
-__device__ __forceinline__ V3<S> lerp_vertex(int x, int y, int z, int axis, S d0, S d1, V3<S> f0, V3<S> f1,
-                                             S iso) {
-    const S t = interp_t(d0, d1, iso);
-    S p0x = S(x), p0y = S(y), p0z = S(z);
-    S p1x = S(x + (axis == 0)), p1y = S(y + (axis == 1)), p1z = S(z + (axis == 2));
-    if (DEF) {
-        p0x += f0.x;
-        p0y += f0.y;
-        p0z += f0.z;
-        p1x += f1.x;
-        p1y += f1.y;
-        p1z += f1.z;
-    }
-    V3<S> r;
-    r.x = p0x + (p1x - p0x) * t;
-    r.y = p0y + (p1y - p0y) * t;
-    r.z = p0z + (p1z - p0z) * t;
-    return r;
-}
-
-// x / c for a divisor c fixed per call, with r = RN(1/c): q = x*r is within an ulp of x/c and
-// one FMA correction with the exact residual makes it the correctly rounded quotient
-// (Markstein's theorem), i.e. bit-identical to IEEE x / c, in 3 instructions instead of the
-// ~10 of a general correctly rounded division.
-template <typename S>
-__device__ __forceinline__ S div_by(S x, S c, S r) {
-    if (c == S(0)) return x / c;  // uniform; keeps IEEE inf/nan for a length-1 axis
-    const S q = x * r;
-    return fma(fma(-q, c, x), r, q);
-}
-
-// Per-axis divisors (dims - 1) and their reciprocals, for the wrapper's normalisation.
-template <typename S>
-struct Norm {
-    S c[3], r[3];
-};
-
-// ------------------------------------------------------------------------------------------
 // ------------------------------------------------------------------------------------------
 // Pass 2: emission
 //
@@ -673,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase
                                                          OutCfg out, Norm<S> nm, int64_t nblocks) {
     __shared__ uint64_t s_case[256];
     __shared__ McEmitLds<S> s_lds[kWavesPerBlock];
-    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     if (TRI) {
         for (int i = tid; i < 256; i += kBlock) s_case[i] = kMcCase[i];
         __syncthreads();
@@ -681,15 +715,14 @@ __global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase
     const Geo &g = in.geo;
     const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
     if (s >= g.nseg) return;
-    const int chunk = (int)(s % g.nchunk);
-    const int64_t row = s / g.nchunk;
-    const int px = (int)(row / g.g1), py = (int)(row % g.g1);
+    const SegPos sp = seg_pos(g, s);
+    const int px = sp.px, py = sp.py, chunk = sp.chunk;
     if (px >= g.g0 - 1 || py >= g.g1 - 1) return;  // no cells on this row (wave-uniform)
     const int z0 = chunk * kWave, pz = z0 + lane;
     const int64_t vbase = seg[s].v, tbase = seg[s].f;
 
-    Win<S, 0, TRI ? 3 : 2, 0, TRI ? 3 : 2, 0, 2> w;
-    w.load(in, px, py, pz);
+    Win<S, 0, TRI ? 3 : 2, 0, TRI ? 3 : 2, 0, 2> w(in, px, py, pz, z0);
+    w.template load_box<0, 2, 0, 2, 0, 2>(in);  // the cell's corners
     const S iso = in.iso;
     const bool anchor = is_anchor(in, px, py, pz);
     const int m00 = anchor ? own_mask(w(0, 0, 0), w(1, 0, 0), w(0, 1, 0), w(0, 0, 1), iso) : 0;
@@ -707,8 +740,12 @@ __global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase
         cw = s_case[code];
     }
     const int ntri = DISO_MC_NTRI(cw);
-    // Most segments lie away from the surface: leave before any ballot.
+    // Most segments lie away from the surface: leave before any ballot or further load.
     if (__ballot(m00 != 0 || ntri != 0) == 0) return;
+    if constexpr (TRI) {
+        w.template load_box<2, 1, 0, 3, 0, 2>(in);
+        w.template load_box<0, 2, 2, 1, 0, 2>(in);
+    }
     int tv, tt = 0;
     const int p00 = wave_excl_small(popc3(m00), &tv);
     const SegBase *tot = seg + g.nseg;
@@ -858,22 +895,21 @@ __global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBas
                                                           OutCfg out, Norm<S> nm, int64_t nblocks) {
     __shared__ DmcTables<S> T;
     __shared__ DmcEmitLds<S> s_lds[kWavesPerBlock];
-    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     load_dmc_tables(T, tid);
     __syncthreads();
     const Geo &g = in.geo;
     const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
     if (s >= g.nseg) return;
-    const int chunk = (int)(s % g.nchunk);
-    const int64_t row = s / g.nchunk;
-    const int px = (int)(row / g.g1), py = (int)(row % g.g1);
+    const SegPos sp = seg_pos(g, s);
+    const int px = sp.px, py = sp.py, chunk = sp.chunk;
     if (px >= g.g0 - 1 || py >= g.g1 - 1) return;
     const int z0 = chunk * kWave, pz = z0 + lane;
     const S iso = in.iso;
     const int64_t qbase = seg[s].v, pbase = seg[s].f;
 
-    Win<S, -1, 4, -1, 4, -1, 3> w;  // x, y in [-1, 2], z in [-1, 1]
-    w.load(in, px, py, pz);
+    Win<S, -1, 4, -1, 4, -1, 3> w(in, px, py, pz, z0);  // x, y in [-1, 2], z in [-1, 1]
+    w.template load_box<0, 2, 0, 2, 0, 2>(in);           // the cell's corners
     const bool anchor = is_anchor(in, px, py, pz);
     const int m00 = anchor ? own_mask(w(0, 0, 0), w(1, 0, 0), w(0, 1, 0), w(0, 0, 1), iso) : 0;
     {
@@ -888,6 +924,11 @@ __global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBas
         }
         if (__ballot(m00 != 0 || (anchor && lo && hi)) == 0) return;
     }
+    w.template load_box<-1, 1, -1, 4, -1, 3>(in);
+    w.template load_box<2, 1, -1, 4, -1, 3>(in);
+    w.template load_box<0, 2, -1, 1, -1, 3>(in);
+    w.template load_box<0, 2, 2, 1, -1, 3>(in);
+    w.template load_box<0, 2, 0, 2, -1, 1>(in);
     const DmcCells cells = dmc_cells(in, T.cases, seg, w, px, py, pz, chunk);
     const uint64_t cw = T.cases[cells.c[0].code];
     const int np = DISO_DMC_NPATCH(cw);
@@ -1090,44 +1131,53 @@ __global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase 
                                                         S *__restrict__ adj_sdf, S *__restrict__ adj_deform,
                                                         int64_t nblocks) {
     __shared__ BwdLds<S> s_lds[kWavesPerBlock];
-    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     const Geo &g = in.geo;
     const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
     if (s >= g.nseg) return;
-    const int chunk = (int)(s % g.nchunk);
-    const int64_t row = s / g.nchunk;
-    const int px = (int)(row / g.g1), py = (int)(row % g.g1);
+    const SegPos sp = seg_pos(g, s);
+    const int px = sp.px, py = sp.py, chunk = sp.chunk;
     const int z0 = chunk * kWave, pz = z0 + lane;
     const S iso = in.iso;
     const int64_t vbase = seg[s].v;
     const int64_t bm0 = px > 0 ? seg[seg_index(g, px - 1, py, chunk)].v : 0;
     const int64_t b0m = py > 0 ? seg[seg_index(g, px, py - 1, chunk)].v : 0;
 
+    // The vertex, its six axis neighbours, and (once there is work) the samples that complete the
+    // crossing masks of the -x, -y, -z owners.
+    Win<S, -1, 3, -1, 3, -1, 3> w(in, px, py, pz, z0);
+    w.template load_box<0, 1, 0, 1, -1, 3>(in);
+    w.template load_box<-1, 1, 0, 1, 0, 1>(in);
+    w.template load_box<1, 1, 0, 1, 0, 1>(in);
+    w.template load_box<0, 1, -1, 1, 0, 1>(in);
+    w.template load_box<0, 1, 1, 1, 0, 1>(in);
     S smp[7];
-    smp[0] = sample(in, px, py, pz);
-    smp[1] = sample(in, px + 1, py, pz);
-    smp[2] = sample(in, px, py + 1, pz);
-    smp[3] = sample(in, px, py, pz + 1);
-    smp[4] = sample(in, px - 1, py, pz);
-    smp[5] = sample(in, px, py - 1, pz);
-    smp[6] = sample(in, px, py, pz - 1);
+    smp[0] = w(0, 0, 0);
+    smp[1] = w(1, 0, 0);
+    smp[2] = w(0, 1, 0);
+    smp[3] = w(0, 0, 1);
+    smp[4] = w(-1, 0, 0);
+    smp[5] = w(0, -1, 0);
+    smp[6] = w(0, 0, -1);
     const S c = smp[0];
     const int m00 = is_anchor(in, px, py, pz) ? own_mask(c, smp[1], smp[2], smp[3], iso) : 0;
-    const int mm0 = is_anchor(in, px - 1, py, pz)
-                        ? own_mask(smp[4], c, sample(in, px - 1, py + 1, pz), sample(in, px - 1, py, pz + 1), iso)
-                        : 0;
-    const int m0m = is_anchor(in, px, py - 1, pz)
-                        ? own_mask(smp[5], sample(in, px + 1, py - 1, pz), c, sample(in, px, py - 1, pz + 1), iso)
-                        : 0;
-    const int m00m = is_anchor(in, px, py, pz - 1)
-                         ? own_mask(smp[6], sample(in, px + 1, py, pz - 1), sample(in, px, py + 1, pz - 1), c, iso)
-                         : 0;
-    const int mask = in_data(in, px, py, pz) ? (m00 | ((mm0 & 1) << 3) | ((m0m & 2) << 3) | ((m00m & 4) << 3)) : 0;
+    const int inc_m = (is_anchor(in, px - 1, py, pz) && crosses(smp[4], c, iso)) |
+                      ((is_anchor(in, px, py - 1, pz) && crosses(smp[5], c, iso)) << 1) |
+                      ((is_anchor(in, px, py, pz - 1) && crosses(smp[6], c, iso)) << 2);
+    const int mask = in_data(in, px, py, pz) ? (m00 | (inc_m << 3)) : 0;
     Grad<S> acc = {S(0), S(0), S(0), S(0)};
     if (__ballot(mask != 0) == 0) {  // no crossing edge touches this segment: zero gradients
         store_grad(in, adj_sdf, adj_deform, px, py, pz, acc, adj.flags);
         return;
     }
+    w.template load_box<-1, 1, 1, 1, 0, 1>(in);
+    w.template load_box<-1, 1, 0, 1, 1, 1>(in);
+    w.template load_box<1, 1, -1, 1, 0, 1>(in);
+    w.template load_box<0, 1, -1, 1, 1, 1>(in);
+    w.template load_box<1, 1, 0, 1, -1, 1>(in);
+    w.template load_box<0, 1, 1, 1, -1, 1>(in);
+    const int mm0 = is_anchor(in, px - 1, py, pz) ? own_mask(smp[4], c, w(-1, 1, 0), w(-1, 0, 1), iso) : 0;
+    const int m0m = is_anchor(in, px, py - 1, pz) ? own_mask(smp[5], w(1, -1, 0), c, w(0, -1, 1), iso) : 0;
     int t, total;
     const int32_t r00 = wave_excl_small(popc3(m00), &t);
     const int32_t rm0 = (int32_t)(bm0 - vbase) + wave_excl_small(popc3(mm0), &t);
@@ -1249,8 +1299,8 @@ bool make_geo(int64_t X, int64_t Y, int64_t Z, int flags, Geo *g) {
     g->off = pad / 2;
     g->nchunk = (g->g2 + kWave - 1) / kWave;
     g->nseg = (int64_t)g->g0 * g->g1 * g->nchunk;
-    // Per-block aggregates must fit the 29/30-bit fields of the scan words.
-    return (int64_t)g->g0 * g->g1 * g->g2 < (1ll << 36);
+    // Per-block aggregates must fit the 29/30-bit fields of the scan words; segment ids 32 bits.
+    return (int64_t)g->g0 * g->g1 * g->g2 < (1ll << 36) && g->nseg < (1ll << 32);
 }
 
 int64_t seg_bytes_for(const Geo &g) { return (g.nseg + 2) * (int64_t)sizeof(SegBase); }
