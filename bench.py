@@ -40,6 +40,7 @@ def parse():
     p.add_argument("--kernel-reps", type=int, default=10)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=3)
+    p.add_argument("--copy-mib", type=int, default=256)
     return p.parse_args()
 
 
@@ -111,7 +112,7 @@ def time_kernels(ex, sdf, deform, flags, g, reps):
 
 
 def algorithmic_bytes(kind, n, s, tv, nv, nf, k):
-    """Bytes a kernel must move (SURVEY byte model, per kernel):
+    """Bytes a kernel must move (the path's byte model, split by kernel):
     count: every sample once; emit: samples + deform at touched vertices, vertices, int64 faces;
     backward: samples + deform at touched vertices, upstream vertex gradients, dense sdf and deform
     gradients."""
@@ -121,6 +122,44 @@ def algorithmic_bytes(kind, n, s, tv, nv, nf, k):
     if kind == "emit":
         return s * tv + 3 * s * tv + 3 * s * nv + 8 * k * nf
     return s * tv + 3 * s * tv + 3 * s * nv + s * n3 + 3 * s * n3
+
+
+def copy_calibration(dev, mib, reps):
+    """A plain elementwise copy (torch.neg, fp32) of `mib` MiB in and out: its duration is the
+    practical HBM streaming rate, and under rocprofv3 its FETCH_SIZE / WRITE_SIZE readings
+    (known to be `mib` MiB each) calibrate the counters' units."""
+    import torch
+
+    n = (mib << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        torch.neg(src, out=dst)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"kernel": "torch.neg fp32", "bytes_read": n * 4, "bytes_written": n * 4, "ms": round(ms, 4),
+            "GBs": round(2 * n * 4 / (ms / 1e3) / 1e9, 1)}
+
+
+def profiled_traffic(phase, n):
+    """HBM bytes per launch of `phase` measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
+    this workload (calibrated against a plain copy), from the committed summary; None when there
+    is no summary for this workload."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_bench_n{n}.txt")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        f = ln.split()
+        if len(f) > 2 and f[0] == "traffic" and f[1] == phase:
+            kv = dict(x.split("=", 1) for x in f[2:] if "=" in x)
+            return int(kv["total_bytes"])
+    return None
 
 
 def cpu_baseline(sdf_np, deform_np, n, seed, reps):
@@ -219,6 +258,7 @@ def main():
     timed = {k: v for k, v in kern.items() if "ms" in v}
     dom = max(timed, key=lambda k: timed[k]["ms"])
     d = timed[dom]
+    copy = copy_calibration(dev, args.copy_mib, args.kernel_reps)
 
     out = {
         "metric": "Gvoxel/s fwd+bwd DiffMC/DMC 256^3 fp32, batched vs plain; p50/p99 ms",
@@ -239,8 +279,10 @@ def main():
                                f"deform U(-0.3,0.3), N={n}", "grid": n, "grids_per_step": 2,
                    "mode": "plain per-grid API", "touched_vertices": tv},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(d["GBs"] / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(d["GBs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, n),
+                     "traffic_source": f"profiles/hbm_traffic_bench_n{n}.txt"},
         "kernels": kern,
+        "copy_calibration": copy,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sets[0][3], sets[0][4], n, args.seed, args.cpu_reps)

@@ -14,13 +14,12 @@ by-product of the counting pass, so a forward costs one host synchronisation (to
 outputs) and a backward none.
 """
 import torch
-import torch.nn.functional as F
 from torch import nn
 
 from . import _lib
-from .engine import Extractor
+from .engine import Extractor, split_quads
 
-__all__ = ["DiffMC", "DiffDMC"]
+__all__ = ["DiffMC", "DiffDMC", "split_quads"]
 __version__ = "0.1.0"
 
 
@@ -65,7 +64,7 @@ class _Extract(torch.autograd.Function):
         return grad_grid, grad_deform, None, None, None
 
 
-def _run(ex, grid, deform, isovalue, normalize, compat, empty_k):
+def _run(ex, grid, deform, isovalue, normalize, compat, recompute, empty_k):
     _check_inputs(ex, grid, deform)
     grid = grid.contiguous()
     deform = None if deform is None else deform.contiguous()
@@ -75,6 +74,8 @@ def _run(ex, grid, deform, isovalue, normalize, compat, empty_k):
         flags |= _lib.FLAG_NORMALIZE
     if compat:
         flags |= _lib.FLAG_REFERENCE_COMPAT
+    if recompute:
+        flags |= _lib.FLAG_RECOMPUTE
     with torch.cuda.device(grid.device):
         counted = ex.count(grid, iso, flags)
         if counted.has_nan:
@@ -88,42 +89,21 @@ def _run(ex, grid, deform, isovalue, normalize, compat, empty_k):
 
 
 class DiffMC(nn.Module):
-    """Differentiable Marching Cubes (diso/__init__.py:9-61)."""
+    """Differentiable Marching Cubes (diso/__init__.py:9-61).
 
-    def __init__(self, dtype=torch.float32):
+    recompute_backward=True keeps only the segment bases between forward and backward (16 bytes
+    per 64 grid positions instead of 2 bytes per position) and re-derives the crossing masks from
+    the samples in the backward; results are identical."""
+
+    def __init__(self, dtype=torch.float32, recompute_backward=False):
         super().__init__()
         self.dtype = dtype
+        self.recompute_backward = recompute_backward
         self._ex = Extractor(_lib.MC, dtype)
 
     def forward(self, grid, deform=None, isovalue=0.0, normalize=True):
-        verts, faces, _ = _run(self._ex, grid, deform, isovalue, normalize, False, 3)
+        verts, faces, _ = _run(self._ex, grid, deform, isovalue, normalize, False, self.recompute_backward, 3)
         return verts, faces
-
-
-def split_quads(verts, quads):
-    """Cuts each quad into two triangles along the diagonal that keeps the smallest angle
-    largest, exactly as diso/__init__.py:117-147: the score of a cut is the largest corner cosine
-    over its two triangles (F.normalize with eps 1e-12); cut (0,1,3)+(1,2,3) is taken when its
-    score is strictly smaller than that of (0,1,2)+(0,2,3). All quads taking the first cut come
-    first, in quad order, then the others."""
-
-    def worst_cosine(cut):
-        per_tri = []
-        for a, b, c in cut:
-            v0, v1, v2 = verts[quads[:, a]], verts[quads[:, b]], verts[quads[:, c]]
-            c1 = (F.normalize(v1 - v0, dim=-1) * F.normalize(v2 - v0, dim=-1)).sum(-1)
-            c2 = (F.normalize(v2 - v1, dim=-1) * F.normalize(v0 - v1, dim=-1)).sum(-1)
-            c3 = (F.normalize(v0 - v2, dim=-1) * F.normalize(v1 - v2, dim=-1)).sum(-1)
-            per_tri.append(torch.max(torch.stack([c1, c2, c3], dim=-1), dim=-1)[0])
-        return torch.max(torch.stack(per_tri, dim=-1), dim=1)[0]
-
-    with torch.no_grad():
-        s1 = worst_cosine(((0, 1, 3), (1, 2, 3)))
-        s2 = worst_cosine(((0, 1, 2), (0, 2, 3)))
-        first = s1 < s2
-        f1 = quads[first][:, [0, 1, 3, 1, 2, 3]].view(-1, 3)
-        f2 = quads[~first][:, [0, 1, 2, 0, 2, 3]].view(-1, 3)
-        return torch.cat([f1, f2], dim=0)
 
 
 class DiffDMC(nn.Module):
@@ -131,16 +111,19 @@ class DiffDMC(nn.Module):
 
     reference_compat=True reproduces the reference backward's read of each cell's first patch
     gradient for all of the cell's patches (src/cudualmc.cu:990); the default computes the true
-    gradient of the forward (the two agree whenever every cell has at most one patch)."""
+    gradient of the forward (the two agree whenever every cell has at most one patch).
+    recompute_backward as for DiffMC."""
 
-    def __init__(self, dtype=torch.float32, reference_compat=False):
+    def __init__(self, dtype=torch.float32, reference_compat=False, recompute_backward=False):
         super().__init__()
         self.dtype = dtype
         self.reference_compat = reference_compat
+        self.recompute_backward = recompute_backward
         self._ex = Extractor(_lib.DMC, dtype)
 
     def forward(self, grid, deform=None, isovalue=0.0, return_quads=False, normalize=True):
-        verts, quads, empty = _run(self._ex, grid, deform, isovalue, normalize, self.reference_compat, 4)
+        verts, quads, empty = _run(self._ex, grid, deform, isovalue, normalize, self.reference_compat,
+                                   self.recompute_backward, 4)
         if return_quads or empty:
             return verts, quads
         return verts, split_quads(verts, quads)

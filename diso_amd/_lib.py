@@ -29,13 +29,14 @@ FLAG_NORMALIZE = 1 << 2
 FLAG_INDEX64 = 1 << 3
 FLAG_REFERENCE_COMPAT = 1 << 4
 FLAG_ACCUMULATE = 1 << 5
+FLAG_RECOMPUTE = 1 << 6
 TOTAL_HAS_BELOW = 1 << 0
 TOTAL_HAS_ABOVE = 1 << 1
 TOTAL_HAS_NAN = 1 << 2
 
 EXPORTS = [
     "diso_seg_bytes", "diso_ctl_bytes", "diso_scratch_bytes", "diso_count", "diso_emit", "diso_backward",
-    "diso_read_totals", "diso_last_error", "diso_version",
+    "diso_read_totals", "diso_last_error", "diso_version", "diso_split_scratch_bytes", "diso_split",
 ]
 
 
@@ -81,11 +82,14 @@ def lib():
         L.diso_scratch_bytes.restype = i64
         L.diso_scratch_bytes.argtypes = [i32, i32, i64]
         L.diso_emit.restype = i32
-        L.diso_emit.argtypes = [i32, i32, vp, vp, i64, i64, i64, dbl, i32, vp, i64, vp, i64, vp, i64, vp,
-                                vp, i64, vp]
+        L.diso_emit.argtypes = [i32, i32, vp, vp, i64, i64, i64, dbl, i32, vp, i64, vp, i64, vp, i64, vp, vp]
         L.diso_backward.restype = i32
         L.diso_backward.argtypes = [i32, i32, vp, vp, i64, i64, i64, dbl, i32, vp, i64, vp, i64, vp, vp,
                                     i64, vp, vp, vp, i64, vp]
+        L.diso_split_scratch_bytes.restype = i64
+        L.diso_split_scratch_bytes.argtypes = [i64]
+        L.diso_split.restype = i32
+        L.diso_split.argtypes = [i32, vp, i64, vp, i64, i32, vp, i64, vp, i64, vp]
         L.diso_read_totals.restype = i32
         L.diso_read_totals.argtypes = [vp, i64, i64, i64, i64, i32, ctypes.POINTER(Totals), vp]
         L.diso_last_error.restype = ctypes.c_char_p

@@ -232,6 +232,20 @@ __device__ __forceinline__ int64_t seg_index(const Geo &g, int px, int py, int c
     return ((int64_t)px * g.g1 + py) * g.nchunk + chunk;
 }
 
+// Position classification word, written by the count pass for every lane of every segment
+// (segment-major: word s * 64 + lane), read by the later passes instead of re-deriving it from
+// the samples: bits 0-7 the code of the cell anchored at the position (MC: Bourke corner order;
+// DMC: Morton corner order after the face-neighbour rule), bits 8-10 the crossing mask of the
+// three edges the position owns (+x, +y, +z), bit 11 whether its own sample is >= iso. Bits 0-10
+// are 0 where no cell is anchored.
+__device__ __forceinline__ int info_code(uint32_t w) { return (int)(w & 0xffu); }
+__device__ __forceinline__ int info_mask(uint32_t w) { return (int)((w >> 8) & 7u); }
+__device__ __forceinline__ int info_inside(uint32_t w) { return (int)((w >> 11) & 1u); }
+
+__device__ __forceinline__ const uint16_t *info_of(const SegBase *seg, const Geo &g) {
+    return reinterpret_cast<const uint16_t *>(seg + g.nseg + 2);
+}
+
 // ------------------------------------------------------------------------------------------
 // Register windows and edge geometry shared by the emission and backward kernels
 
@@ -306,19 +320,6 @@ __device__ __forceinline__ int own_mask(S d0, S dx, S dy, S dz, S iso) {
     return crosses(d0, dx, iso) | (crosses(d0, dy, iso) << 1) | (crosses(d0, dz, iso) << 2);
 }
 
-// DMC code of the cell anchored at window offset (cx,cy,cz) (corners cx..cx+1 etc.), resolved
-// with the face-neighbour rule; 0 when the position is not a cell.
-template <typename S, typename W>
-__device__ __forceinline__ int dmc_code_w(const In<S> &in, const uint64_t *cases, const W &w, int px, int py,
-                                          int pz, int cx, int cy, int cz) {
-    if (!is_anchor(in, px + cx, py + cy, pz + cz)) return 0;
-    int code = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) code |= (w(cx + (i & 1), cy + ((i >> 1) & 1), cz + ((i >> 2) & 1)) >= in.iso) << i;
-    if (code == 0 || code == 255) return code;
-    return dmc_good_code(in, cases, code, px + cx, py + cy, pz + cz);
-}
-
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -351,6 +352,14 @@ __device__ __forceinline__ V3<S> lerp_vertex(int x, int y, int z, int axis, S d0
     r.y = p0y + (p1y - p0y) * t;
     r.z = p0z + (p1z - p0z) * t;
     return r;
+}
+
+// MC vertex on the edge from (x,y,z) along `axis`: samples and deforms of both endpoints.
+template <typename S, bool DEF>
+__device__ __forceinline__ V3<S> edge_vertex(const In<S> &in, int x, int y, int z, int axis) {
+    const int x1 = x + (axis == 0), y1 = y + (axis == 1), z1 = z + (axis == 2);
+    return lerp_vertex<S, DEF>(x, y, z, axis, sample(in, x, y, z), sample(in, x1, y1, z1),
+                               deform_at<S, DEF>(in, x, y, z), deform_at<S, DEF>(in, x1, y1, z1), in.iso);
 }
 
 // x / c for a divisor c fixed per call, with r = RN(1/c): q = x*r is within an ulp of x/c and
@@ -408,32 +417,40 @@ __device__ __forceinline__ uint64_t pack_agg(int64_t v, int64_t f, uint32_t fl) 
 }
 
 // Per-position counts for pass 1: nv = crossing edges owned (MC vertices / DMC quads);
-// nf = MC triangles / DMC patches; fl = sample flags of the position's own sample.
+// nf = MC triangles / DMC patches; fl = sample flags of the position's own sample; the position's
+// classification word.
+template <typename S>
+using CornerWin = Win<S, 0, 2, 0, 2, 0, 2>;
+
 template <typename S, int ALGO>
-__device__ __forceinline__ void count_position(const In<S> &in, const uint64_t *cases, int px, int py, int pz,
-                                               int z0, int *nv, int *nf, uint32_t *fl) {
-    Win<S, 0, 2, 0, 2, 0, 2> w(in, px, py, pz, z0);
-    w.template load_box<0, 2, 0, 2, 0, 2>(in);
+__device__ __forceinline__ void count_position(const In<S> &in, const uint64_t *cases, const CornerWin<S> &w,
+                                               int *nv, int *nf, uint32_t *fl, uint32_t *info) {
+    const int px = w.px, py = w.py, pz = w.pz;
     const S v = w(0, 0, 0);
     *fl = in_data(in, px, py, pz) ? ((v < in.iso ? DISO_TOTAL_HAS_BELOW : 0u) |
                                      (v > in.iso ? DISO_TOTAL_HAS_ABOVE : 0u) | (v != v ? DISO_TOTAL_HAS_NAN : 0u))
                                   : 0u;
     *nv = 0;
     *nf = 0;
+    *info = (uint32_t)(v >= in.iso) << 11;
     if (!is_anchor(in, px, py, pz)) return;
-    *nv = crosses(v, w(1, 0, 0), in.iso) + crosses(v, w(0, 1, 0), in.iso) + crosses(v, w(0, 0, 1), in.iso);
+    const int m = own_mask(v, w(1, 0, 0), w(0, 1, 0), w(0, 0, 1), in.iso);
+    *nv = popc3(m);
+    int code;
     if (ALGO == DISO_MC) {
         Corners<S> k;
 #pragma unroll
         for (int i = 0; i < 8; ++i) k.c[i & 1][(i >> 1) & 1][(i >> 2) & 1] = w(i & 1, (i >> 1) & 1, (i >> 2) & 1);
-        *nf = DISO_MC_NTRI(cases[mc_code(k, in.iso)]);
+        code = mc_code(k, in.iso);
+        *nf = DISO_MC_NTRI(cases[code]);
     } else {
-        int code = 0;
+        code = 0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) code |= (w(i & 1, (i >> 1) & 1, (i >> 2) & 1) >= in.iso) << i;
         if (code != 0 && code != 255) code = dmc_good_code(in, cases, code, px, py, pz);
         *nf = DISO_DMC_NPATCH(cases[code]);
     }
+    *info |= (uint32_t)code | ((uint32_t)m << 8);
 }
 
 // Wave sum of a per-lane value in [0, 7] (three ballots).
@@ -476,22 +493,37 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *s
     const uint32_t parity = s_parity;
     uint64_t *cur = words + (int64_t)parity * kMaxCountBlocks;
 
-    // 1. Count: wave w takes segments lo+w, lo+w+8, ...; lane 0 records each segment's counts.
+    // 1. Count: wave w takes segments lo+w, lo+w+8, ...; lane 0 records each segment's counts,
+    // every lane its classification word.
+    uint16_t *info = const_cast<uint16_t *>(info_of(seg, g));
     int64_t wv = 0, wf = 0;
     uint32_t wfl = 0;
-    for (int64_t s = lo + wave; s < hi; s += kCountWaves) {  // wave-uniform
-        const SegPos sp = seg_pos(g, s);
-        int nv, nf;
-        uint32_t fl;
-        count_position<S, ALGO>(in, s_case, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave, &nv, &nf, &fl);
-        const int tv = wave_total_small(nv), tf = wave_total_small(nf);
-        if (lane == 0) {
-            seg[s].v = tv;
-            seg[s].f = tf;
+    // Two segments per iteration, both windows loaded before either is classified, so that two
+    // memory round trips overlap.
+    for (int64_t s0 = lo + wave; s0 < hi; s0 += 2 * kCountWaves) {  // wave-uniform
+        const int64_t s1 = s0 + kCountWaves < hi ? s0 + kCountWaves : s0;
+        const SegPos a = seg_pos(g, s0), b = seg_pos(g, s1);
+        CornerWin<S> wa(in, a.px, a.py, a.chunk * kWave + lane, a.chunk * kWave);
+        CornerWin<S> wb(in, b.px, b.py, b.chunk * kWave + lane, b.chunk * kWave);
+        wa.template load_box<0, 2, 0, 2, 0, 2>(in);
+        wb.template load_box<0, 2, 0, 2, 0, 2>(in);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t s = h ? s1 : s0;
+            if (h && s1 == s0) break;
+            int nv, nf;
+            uint32_t fl, iw;
+            count_position<S, ALGO>(in, s_case, h ? wb : wa, &nv, &nf, &fl, &iw);
+            info[s * kWave + lane] = (uint16_t)iw;
+            const int tv = wave_total_small(nv), tf = wave_total_small(nf);
+            if (lane == 0) {
+                seg[s].v = tv;
+                seg[s].f = tf;
+            }
+            wv += tv;
+            wf += tf;
+            wfl |= fl;
         }
-        wv += tv;
-        wf += tf;
-        wfl |= fl;
     }
     wfl = wave_or(wfl);
     if (lane == 0) {
@@ -677,166 +709,90 @@ __device__ __forceinline__ void edge_ids(int32_t *c, int32_t r00, int32_t r10, i
     c[11] = r00 + n00 + c00n;
 }
 
-// Crossing masks of the rows R10, R01, R11 and +z flags for edge_ids, from a window that covers
-// offsets x, y in [0, 2] and z in [0, 1].
-template <typename S, typename W>
-__device__ __forceinline__ void neighbour_masks(const In<S> &in, const W &w, int px, int py, int pz, int *m10,
-                                                int *m01, int *m11, int *c00n, int *c10n) {
-    const S iso = in.iso;
-    *m10 = is_anchor(in, px + 1, py, pz) ? own_mask(w(1, 0, 0), w(2, 0, 0), w(1, 1, 0), w(1, 0, 1), iso) : 0;
-    *m01 = is_anchor(in, px, py + 1, pz) ? own_mask(w(0, 1, 0), w(1, 1, 0), w(0, 2, 0), w(0, 1, 1), iso) : 0;
-    *m11 = is_anchor(in, px + 1, py + 1, pz) ? own_mask(w(1, 1, 0), w(2, 1, 0), w(1, 2, 0), w(1, 1, 1), iso) : 0;
-    *c00n = is_anchor(in, px, py, pz + 1) ? crosses(w(0, 0, 1), w(1, 0, 1), iso) : 0;
-    *c10n = is_anchor(in, px + 1, py, pz + 1) ? crosses(w(1, 0, 1), w(2, 0, 1), iso) : 0;
-}
-
-// MC emission (TRI) or MC edge vertices only (!TRI, for DMC). Vertex item = (lane, axis) of an
-// owned crossing edge, in the reference's order (cell, then axis x < y < z;
-// src/cumc.cu:370-410); triangle corner item = (lane, table edge).
-template <typename S>
+// MC emission. Vertex item = (lane, axis) of an owned crossing edge, in the reference's order
+// (cell, then axis x < y < z; src/cumc.cu:370-410); triangle corner item = (lane, table edge).
+// Cell codes and crossing masks come from the count pass's classification words.
 struct McEmitLds {
-    S d[4][kWave];            // sample, +x, +y, +z sample of each lane's cell
     int32_t cand[kWave][12];  // vertex id of each table edge, relative to the segment's first vertex
     uint16_t corner[kWave * 15];
     uint8_t vert[kWave * 3];
 };
 
-template <typename S, bool DEF, bool TRI>
+template <typename S, bool DEF>
 __global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase *__restrict__ seg,
                                                          S *__restrict__ verts, void *__restrict__ faces,
                                                          OutCfg out, Norm<S> nm, int64_t nblocks) {
     __shared__ uint64_t s_case[256];
-    __shared__ McEmitLds<S> s_lds[kWavesPerBlock];
+    __shared__ McEmitLds s_lds[kWavesPerBlock];
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
-    if (TRI) {
-        for (int i = tid; i < 256; i += kBlock) s_case[i] = kMcCase[i];
-        __syncthreads();
-    }
+    for (int i = tid; i < 256; i += kBlock) s_case[i] = kMcCase[i];
+    __syncthreads();
     const Geo &g = in.geo;
     const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
     if (s >= g.nseg) return;
+    const int64_t tbase = seg[s].f;
+    if (seg[s + 1].f == tbase) return;  // no triangle (hence no owned crossing) in this segment
     const SegPos sp = seg_pos(g, s);
     const int px = sp.px, py = sp.py, chunk = sp.chunk;
-    if (px >= g.g0 - 1 || py >= g.g1 - 1) return;  // no cells on this row (wave-uniform)
     const int z0 = chunk * kWave, pz = z0 + lane;
-    const int64_t vbase = seg[s].v, tbase = seg[s].f;
-
-    Win<S, 0, TRI ? 3 : 2, 0, TRI ? 3 : 2, 0, 2> w(in, px, py, pz, z0);
-    w.template load_box<0, 2, 0, 2, 0, 2>(in);  // the cell's corners
-    const S iso = in.iso;
-    const bool anchor = is_anchor(in, px, py, pz);
-    const int m00 = anchor ? own_mask(w(0, 0, 0), w(1, 0, 0), w(0, 1, 0), w(0, 0, 1), iso) : 0;
-    uint64_t cw = 0;
-    if (TRI && anchor) {  // Bourke corner order (src/cumc.cu:107)
-        int code = 0;
-        code |= (w(0, 0, 0) >= iso) << 0;
-        code |= (w(1, 0, 0) >= iso) << 1;
-        code |= (w(1, 1, 0) >= iso) << 2;
-        code |= (w(0, 1, 0) >= iso) << 3;
-        code |= (w(0, 0, 1) >= iso) << 4;
-        code |= (w(1, 0, 1) >= iso) << 5;
-        code |= (w(1, 1, 1) >= iso) << 6;
-        code |= (w(0, 1, 1) >= iso) << 7;
-        cw = s_case[code];
-    }
-    const int ntri = DISO_MC_NTRI(cw);
-    // Most segments lie away from the surface: leave before any ballot or further load.
-    if (__ballot(m00 != 0 || ntri != 0) == 0) return;
-    if constexpr (TRI) {
-        w.template load_box<2, 1, 0, 3, 0, 2>(in);
-        w.template load_box<0, 2, 2, 1, 0, 2>(in);
-    }
-    int tv, tt = 0;
-    const int p00 = wave_excl_small(popc3(m00), &tv);
+    const int64_t vbase = seg[s].v;
     const SegBase *tot = seg + g.nseg;
-    if (tot[0].v > out.vcap || (TRI && tot[0].f > out.fcap)) return;
+    if (tot[0].v > out.vcap || tot[0].f > out.fcap) return;
 
-    McEmitLds<S> &L = s_lds[wave];
-    L.d[0][lane] = w(0, 0, 0);
-    L.d[1][lane] = w(1, 0, 0);
-    L.d[2][lane] = w(0, 1, 0);
-    L.d[3][lane] = w(0, 0, 1);
+    // This row has cells, so rows +x, +y, +x+y exist; +z words are read within the row only.
+    const uint16_t *info = info_of(seg, g);
+    const int64_t s10 = seg_index(g, px + 1, py, chunk), s01 = seg_index(g, px, py + 1, chunk),
+                  s11 = seg_index(g, px + 1, py + 1, chunk);
+    const uint32_t w00 = info[s * kWave + lane];
+    const int m00 = info_mask(w00);
+    const int m10 = info_mask(info[s10 * kWave + lane]);
+    const int m01 = info_mask(info[s01 * kWave + lane]);
+    const int m11 = info_mask(info[s11 * kWave + lane]);
+    const bool zn = pz + 1 < g.g2;
+    const int c00n = zn ? info_mask(info[s * kWave + lane + 1]) & 1 : 0;
+    const int c10n = zn ? info_mask(info[s10 * kWave + lane + 1]) & 1 : 0;
+    const uint64_t cw = s_case[info_code(w00)];  // Bourke corner order (src/cumc.cu:107)
+    const int ntri = DISO_MC_NTRI(cw);
+
+    int tv, tt, t;
+    const int p00 = wave_excl_small(popc3(m00), &tv);
+    const int pt = wave_excl_small(ntri, &tt);
+    const int32_t r10 = (int32_t)(seg[s10].v - vbase) + wave_excl_small(popc3(m10), &t);
+    const int32_t r01 = (int32_t)(seg[s01].v - vbase) + wave_excl_small(popc3(m01), &t);
+    const int32_t r11 = (int32_t)(seg[s11].v - vbase) + wave_excl_small(popc3(m11), &t);
+
+    McEmitLds &L = s_lds[wave];
     {
         int r = p00;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             if ((m00 >> a) & 1) L.vert[r++] = (uint8_t)(lane * 4 + a);
     }
-    if (TRI) {
-        int m10, m01, m11, c00n, c10n, t;
-        neighbour_masks(in, w, px, py, pz, &m10, &m01, &m11, &c00n, &c10n);
-        const int pt = wave_excl_small(ntri, &tt);
-        const int32_t r10 = (int32_t)(seg[seg_index(g, px + 1, py, chunk)].v - vbase) + wave_excl_small(popc3(m10), &t);
-        const int32_t r01 = (int32_t)(seg[seg_index(g, px, py + 1, chunk)].v - vbase) + wave_excl_small(popc3(m01), &t);
-        const int32_t r11 = (int32_t)(seg[seg_index(g, px + 1, py + 1, chunk)].v - vbase) + wave_excl_small(popc3(m11), &t);
-        if (ntri) {
-            edge_ids(L.cand[lane], p00, r10, r01, r11, m00, m10, m01, m11, c00n, c10n);
-            for (int i = 0; i < 3 * ntri; ++i) L.corner[3 * pt + i] = (uint16_t)((lane << 4) | DISO_MC_EDGE(cw, i));
-        }
+    if (ntri) {
+        edge_ids(L.cand[lane], p00, r10, r01, r11, m00, m10, m01, m11, c00n, c10n);
+        for (int i = 0; i < 3 * ntri; ++i) L.corner[3 * pt + i] = (uint16_t)((lane << 4) | DISO_MC_EDGE(cw, i));
     }
     wave_lds_sync();
 
     for (int k = lane; k < tv; k += kWave) {
         const int it = L.vert[k], src = it >> 2, a = it & 3;
-        const int z = z0 + src;
-        const V3<S> f0 = deform_at<S, DEF>(in, px, py, z);
-        const V3<S> f1 = deform_at<S, DEF>(in, px + (a == 0), py + (a == 1), z + (a == 2));
-        const V3<S> p = lerp_vertex<S, DEF>(px, py, z, a, L.d[0][src], L.d[1 + a][src], f0, f1, iso);
+        const V3<S> p = edge_vertex<S, DEF>(in, px, py, z0 + src, a);
         store_vertex(verts, vbase + k, out.vcap, epilogue(p, out.flags, nm));
     }
-    if (TRI) {
-        const int64_t f0 = 3 * tbase, flim = 3 * out.fcap;
-        for (int k = lane; k < 3 * tt; k += kWave) {
-            const int it = L.corner[k];
-            if (f0 + k < flim) store_index(faces, f0 + k, vbase + L.cand[it >> 4][it & 15], out.flags);
-        }
+    const int64_t f0 = 3 * tbase, flim = 3 * out.fcap;
+    for (int k = lane; k < 3 * tt; k += kWave) {
+        const int it = L.corner[k];
+        if (f0 + k < flim) store_index(faces, f0 + k, vbase + L.cand[it >> 4][it & 15], out.flags);
     }
 }
 
-// DMC emission from the MC edge vertices `mcv` (written by emit_mc_kernel<.., false> in MC
-// vertex order). Patch-vertex item = (lane, patch): the mean of the patch's MC edge vertices,
-// summed from zero in ascending edge order and scaled by 1/num (src/cudualmc.cu:907-955) - the
-// same values the reference recomputes per cell, read once from mcv. Quad item = (lane, axis,
-// leaving) of an owned crossing edge (src/cudualmc.cu:1027-1056): its corners are the vertices
-// of the patches holding the edge in the four cells around it, which lie in rows R00, Rm0 (-x),
-// R0m (-y), Rmm (-x-y) at this z and z-1.
-struct DmcCell {
-    int code;      // resolved code (0 when not a cell)
-    int64_t base;  // first DMC vertex of the cell
-};
-
-__device__ __forceinline__ int64_t row_patch_base(const SegBase *__restrict__ seg, const Geo &g, int px, int py,
-                                                  int chunk) {
-    return (px >= 0 && py >= 0) ? seg[seg_index(g, px, py, chunk)].f : 0;
-}
-
-// The cells (cx,cy,cz) in {-1,0}^3 around a lane's position, indexed ci = 4*(-cx) + 2*(-cy) + (-cz)
-// (cell 7 is not computed): codes from the window, first-vertex ids from the segment bases and
-// the in-wave prefix of patch counts.
-struct DmcCells {
-    DmcCell c[8];
-};
-
-template <typename S, typename W>
-__device__ __forceinline__ DmcCells dmc_cells(const In<S> &in, const uint64_t *cases, const SegBase *__restrict__ seg,
-                                              const W &w, int px, int py, int pz, int chunk) {
-    DmcCells r;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int cur = dmc_code_w(in, cases, w, px, py, pz, -i, -j, 0);
-            int t;
-            const int64_t b =
-                row_patch_base(seg, in.geo, px - i, py - j, chunk) + wave_excl_small(DISO_DMC_NPATCH(cases[cur]), &t);
-            r.c[4 * i + 2 * j].code = cur;
-            r.c[4 * i + 2 * j].base = b;
-            const int prev = (i && j) ? 0 : dmc_code_w(in, cases, w, px, py, pz, -i, -j, -1);
-            r.c[4 * i + 2 * j + 1].code = prev;
-            r.c[4 * i + 2 * j + 1].base = b - DISO_DMC_NPATCH(cases[prev]);
-        }
-    return r;
-}
+// DMC emission. Patch-vertex item = (lane, patch): the mean of the MC vertices on the patch's
+// edges, summed from zero in ascending edge order and scaled by 1/num (src/cudualmc.cu:907-955);
+// each MC vertex is evaluated with the same expression as the MC pass, so it has the same bits.
+// Quad item = (lane, axis, leaving) of an owned crossing edge (src/cudualmc.cu:1027-1056): its
+// corners are the vertices of the patches holding the edge in the four cells around it, which
+// lie in rows R00, Rm0 (-x), R0m (-y), Rmm (-x-y) at this z and z-1. Cell codes come from the
+// count pass's classification words, so no sample is read to classify.
 
 // Quad stencil (src/cudualmc.cu:504-513) as cell index ci and edge id: entry k of type ty.
 // Types 3..5 (edges leaving the inside) list the cells of types 0..2 in the order 0,3,2,1.
@@ -849,21 +805,39 @@ __device__ constexpr int stencil_eid(int ty, int k) {
     return ty < 3 ? q[ty][k] : q[ty - 3][(4 - k) & 3];
 }
 
-template <typename S>
 struct DmcEmitLds {
-    int32_t cand[kWave][12];   // MC vertex id of each of the cell's edges, relative to the segment
-    uint8_t code[8][kWave];    // resolved codes of the cells around the lane
+    uint8_t code[8][kWave];    // codes of the cells (-i,-j,-k) around the lane, ci = 4i + 2j + k
     int32_t base[8][kWave];    // their first DMC vertex, relative to the segment's first
     uint8_t patch[kWave * 4];  // patch items: lane * 4 + patch
     uint16_t quad[kWave * 3];  // quad items: lane * 8 + axis * 2 + leaving
 };
 
-// Block-wide DMC tables: case words, edge mask of each (case, patch), the quad stencil and the
-// reciprocals 1/n of the possible patch sizes.
+// Edge masks of the patches of every DMC case, 16 bits per patch (patch k in bits 16k..16k+11),
+// derived at compile time from the case table.
+struct PatchEdgeTable {
+    uint64_t m[256];
+};
+constexpr PatchEdgeTable make_patch_edges() {
+    const uint64_t cases[256] = {DISO_DMC_CASE_VALUES};
+    PatchEdgeTable t{};
+    for (int c = 0; c < 256; ++c) {
+        uint64_t w = 0;
+        for (int e = 0; e < 12; ++e) {
+            const int k = DISO_DMC_PATCH_OF(cases[c], e);
+            if (k < 4) w |= 1ull << (16 * k + e);
+        }
+        t.m[c] = w;
+    }
+    return t;
+}
+__constant__ PatchEdgeTable kDmcPatchEdges = make_patch_edges();
+
+// Block-wide DMC tables: case words, patch edge masks, the quad stencil and the reciprocals 1/n
+// of the possible patch sizes.
 template <typename S>
 struct DmcTables {
     uint64_t cases[256];
-    uint16_t pedges[256][4];
+    uint64_t pedges[256];
     S rcp[8];
     uint8_t st_ci[6][4], st_eid[6][4];
 };
@@ -871,13 +845,8 @@ struct DmcTables {
 template <typename S>
 __device__ __forceinline__ void load_dmc_tables(DmcTables<S> &T, int tid) {
     for (int i = tid; i < 256; i += kBlock) {
-        const uint64_t w = kDmcCase[i];
-        T.cases[i] = w;
-        for (int k = 0; k < 4; ++k) {
-            uint16_t m = 0;
-            for (int e = 0; e < 12; ++e) m |= (uint16_t)((DISO_DMC_PATCH_OF(w, e) == k) << e);
-            T.pedges[i][k] = m;
-        }
+        T.cases[i] = kDmcCase[i];
+        T.pedges[i] = kDmcPatchEdges.m[i];
     }
     if (tid < 8) T.rcp[tid] = tid ? S(1) / S(tid) : S(0);
     if (tid < 24) {
@@ -887,100 +856,77 @@ __device__ __forceinline__ void load_dmc_tables(DmcTables<S> &T, int tid) {
     }
 }
 
-template <typename S>
+template <typename S, bool DEF>
 __global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBase *__restrict__ seg,
-                                                          const S *__restrict__ mcv, int64_t mcv_cap,
-                                                          S *__restrict__ verts,
-                                                          void *__restrict__ faces, uint8_t *__restrict__ vinfo,
-                                                          OutCfg out, Norm<S> nm, int64_t nblocks) {
+                                                          S *__restrict__ verts, void *__restrict__ faces,
+                                                          uint8_t *__restrict__ vinfo, OutCfg out, Norm<S> nm,
+                                                          int64_t nblocks) {
     __shared__ DmcTables<S> T;
-    __shared__ DmcEmitLds<S> s_lds[kWavesPerBlock];
+    __shared__ DmcEmitLds s_lds[kWavesPerBlock];
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     load_dmc_tables(T, tid);
     __syncthreads();
     const Geo &g = in.geo;
     const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
     if (s >= g.nseg) return;
+    const int64_t pbase = seg[s].f;
+    if (seg[s + 1].f == pbase) return;  // no patch (hence no owned crossing) in this segment
     const SegPos sp = seg_pos(g, s);
     const int px = sp.px, py = sp.py, chunk = sp.chunk;
-    if (px >= g.g0 - 1 || py >= g.g1 - 1) return;
     const int z0 = chunk * kWave, pz = z0 + lane;
-    const S iso = in.iso;
-    const int64_t qbase = seg[s].v, pbase = seg[s].f;
+    const int64_t qbase = seg[s].v;
+    const uint16_t *info = info_of(seg, g);
+    const uint32_t w0 = info[s * kWave + lane];
+    const SegBase *tot = seg + g.nseg;
+    if (tot[0].v > out.fcap || tot[0].f > out.vcap) return;
 
-    Win<S, -1, 4, -1, 4, -1, 3> w(in, px, py, pz, z0);  // x, y in [-1, 2], z in [-1, 1]
-    w.template load_box<0, 2, 0, 2, 0, 2>(in);           // the cell's corners
-    const bool anchor = is_anchor(in, px, py, pz);
-    const int m00 = anchor ? own_mask(w(0, 0, 0), w(1, 0, 0), w(0, 1, 0), w(0, 0, 1), iso) : 0;
-    {
-        // A cell with patches has mixed corners; leave early when no lane owns a crossing edge
-        // or a patch.
-        bool lo = false, hi = false;
+    DmcEmitLds &L = s_lds[wave];
+    // Codes and first vertices of the cells (-i, -j, 0) and (-i, -j, -1).
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const bool b = w(i & 1, (i >> 1) & 1, (i >> 2) & 1) >= iso;
-            lo |= !b;
-            hi |= b;
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const bool row = px - i >= 0 && py - j >= 0;  // wave-uniform
+            const int64_t r = row ? seg_index(g, px - i, py - j, chunk) : 0;
+            const int cur = (i || j) ? (row ? info_code(info[r * kWave + lane]) : 0) : info_code(w0);
+            const int prev = (row && !(i && j) && pz >= 1) ? info_code(info[r * kWave + lane - 1]) : 0;
+            int t;
+            const int32_t b = (int32_t)((row ? seg[r].f : 0) - pbase) + wave_excl_small(DISO_DMC_NPATCH(T.cases[cur]), &t);
+            L.code[4 * i + 2 * j][lane] = (uint8_t)cur;
+            L.base[4 * i + 2 * j][lane] = b;
+            L.code[4 * i + 2 * j + 1][lane] = (uint8_t)prev;
+            L.base[4 * i + 2 * j + 1][lane] = b - DISO_DMC_NPATCH(T.cases[prev]);
         }
-        if (__ballot(m00 != 0 || (anchor && lo && hi)) == 0) return;
-    }
-    w.template load_box<-1, 1, -1, 4, -1, 3>(in);
-    w.template load_box<2, 1, -1, 4, -1, 3>(in);
-    w.template load_box<0, 2, -1, 1, -1, 3>(in);
-    w.template load_box<0, 2, 2, 1, -1, 3>(in);
-    w.template load_box<0, 2, 0, 2, -1, 1>(in);
-    const DmcCells cells = dmc_cells(in, T.cases, seg, w, px, py, pz, chunk);
-    const uint64_t cw = T.cases[cells.c[0].code];
-    const int np = DISO_DMC_NPATCH(cw);
-    int tq, tp, t;
+    const int m00 = info_mask(w0);
+    const int np = DISO_DMC_NPATCH(T.cases[info_code(w0)]);
+    int tq, tp;
     const int pq = wave_excl_small(popc3(m00), &tq);
     const int pp = wave_excl_small(np, &tp);
-    int m10, m01, m11, c00n, c10n;
-    neighbour_masks(in, w, px, py, pz, &m10, &m01, &m11, &c00n, &c10n);
-    const int32_t r10 = (int32_t)(seg[seg_index(g, px + 1, py, chunk)].v - qbase) + wave_excl_small(popc3(m10), &t);
-    const int32_t r01 = (int32_t)(seg[seg_index(g, px, py + 1, chunk)].v - qbase) + wave_excl_small(popc3(m01), &t);
-    const int32_t r11 = (int32_t)(seg[seg_index(g, px + 1, py + 1, chunk)].v - qbase) + wave_excl_small(popc3(m11), &t);
-    const SegBase *tot = seg + g.nseg;
-    if (tot[0].v > out.fcap || tot[0].f > out.vcap || tot[0].v > mcv_cap) return;
-
-    DmcEmitLds<S> &L = s_lds[wave];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        L.code[i][lane] = (uint8_t)cells.c[i].code;
-        L.base[i][lane] = (int32_t)(cells.c[i].base - pbase);
-    }
-    if (np) {
-        edge_ids(L.cand[lane], pq, r10, r01, r11, m00, m10, m01, m11, c00n, c10n);
-        for (int k = 0; k < np; ++k) L.patch[pp + k] = (uint8_t)(lane * 4 + k);
-    }
+    for (int k = 0; k < np; ++k) L.patch[pp + k] = (uint8_t)(lane * 4 + k);
     {
-        const S d0 = w(0, 0, 0);
+        // an owned crossing edge leaves the inside when the position's sample is >= iso
+        // (type a + 3, src/cudualmc.cu:782-788)
+        const int leaving = info_inside(w0);
         int q = pq;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (!((m00 >> a) & 1)) continue;
-            const S d1 = a == 0 ? w(1, 0, 0) : (a == 1 ? w(0, 1, 0) : w(0, 0, 1));
-            const int leaving = d1 < iso && d0 >= iso;  // type a + 3 (src/cudualmc.cu:782-788)
-            L.quad[q++] = (uint16_t)(lane * 8 + a * 2 + leaving);
-        }
+        for (int a = 0; a < 3; ++a)
+            if ((m00 >> a) & 1) L.quad[q++] = (uint16_t)(lane * 8 + a * 2 + leaving);
     }
     wave_lds_sync();
 
-    const int64_t vlim = tot[0].v;  // MC vertex count (rows of mcv)
     for (int k = lane; k < tp; k += kWave) {
         const int it = L.patch[k], src = it >> 2, pk = it & 3;
-        unsigned em = T.pedges[L.code[0][src]][pk];
+        unsigned em = (unsigned)(T.pedges[L.code[0][src]] >> (16 * pk)) & 0xfffu;
         const int num = __popc(em);
+        const int z = z0 + src;
         S sx = S(0), sy = S(0), sz = S(0);
         while (em) {  // ascending edge order
             const int e = __ffs(em) - 1;
             em &= em - 1;
-            const int64_t v = qbase + L.cand[src][e];
-            if (v >= 0 && v < vlim) {
-                sx += mcv[3 * v + 0];
-                sy += mcv[3 * v + 1];
-                sz += mcv[3 * v + 2];
-            }
+            const V3<S> p = edge_vertex<S, DEF>(in, px + eox(e), py + eoy(e), z + eoz(e), eax(e));
+            sx += p.x;
+            sy += p.y;
+            sz += p.z;
         }
         const S r = T.rcp[num];
         store_vertex(verts, pbase + k, out.vcap, epilogue(V3<S>{sx * r, sy * r, sz * r}, out.flags, nm));
@@ -1114,19 +1060,24 @@ __device__ __forceinline__ void store_grad(const In<S> &in, S *__restrict__ adj_
     }
 }
 
-// Per-wave LDS of the gather: the samples a vertex needs, its own (0), +x +y +z (1..3),
-// -x -y -z (4..6); its deform; the vertex id of each incident edge relative to the segment's
-// first vertex; the item list (lane * 8 + slot) and one round of item results.
+// Per-wave LDS of the gather: each vertex's sample and deform; the vertex id of each incident
+// edge relative to the segment's first vertex; the item list (lane * 8 + slot) and one round of
+// item results.
 template <typename S>
 struct BwdLds {
-    S smp[7][kWave];
+    S c[kWave];
     S fc[3][kWave];
     int32_t id[6][kWave];
     uint16_t item[kWave * 6];
     Grad<S> res[kWave];
 };
 
-template <typename S, bool DEF>
+// Whether segment t (-1: none) owns a crossing edge.
+__device__ __forceinline__ bool seg_owns_crossing(const SegBase *__restrict__ seg, int64_t t) {
+    return t >= 0 && seg[t + 1].v != seg[t].v;
+}
+
+template <typename S, bool DEF, bool RECOMP>
 __global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase *__restrict__ seg, AdjIn<S> adj,
                                                         S *__restrict__ adj_sdf, S *__restrict__ adj_deform,
                                                         int64_t nblocks) {
@@ -1139,55 +1090,53 @@ __global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase 
     const int px = sp.px, py = sp.py, chunk = sp.chunk;
     const int z0 = chunk * kWave, pz = z0 + lane;
     const S iso = in.iso;
-    const int64_t vbase = seg[s].v;
-    const int64_t bm0 = px > 0 ? seg[seg_index(g, px - 1, py, chunk)].v : 0;
-    const int64_t b0m = py > 0 ? seg[seg_index(g, px, py - 1, chunk)].v : 0;
-
-    // The vertex, its six axis neighbours, and (once there is work) the samples that complete the
-    // crossing masks of the -x, -y, -z owners.
-    Win<S, -1, 3, -1, 3, -1, 3> w(in, px, py, pz, z0);
-    w.template load_box<0, 1, 0, 1, -1, 3>(in);
-    w.template load_box<-1, 1, 0, 1, 0, 1>(in);
-    w.template load_box<1, 1, 0, 1, 0, 1>(in);
-    w.template load_box<0, 1, -1, 1, 0, 1>(in);
-    w.template load_box<0, 1, 1, 1, 0, 1>(in);
-    S smp[7];
-    smp[0] = w(0, 0, 0);
-    smp[1] = w(1, 0, 0);
-    smp[2] = w(0, 1, 0);
-    smp[3] = w(0, 0, 1);
-    smp[4] = w(-1, 0, 0);
-    smp[5] = w(0, -1, 0);
-    smp[6] = w(0, 0, -1);
-    const S c = smp[0];
-    const int m00 = is_anchor(in, px, py, pz) ? own_mask(c, smp[1], smp[2], smp[3], iso) : 0;
-    const int inc_m = (is_anchor(in, px - 1, py, pz) && crosses(smp[4], c, iso)) |
-                      ((is_anchor(in, px, py - 1, pz) && crosses(smp[5], c, iso)) << 1) |
-                      ((is_anchor(in, px, py, pz - 1) && crosses(smp[6], c, iso)) << 2);
-    const int mask = in_data(in, px, py, pz) ? (m00 | (inc_m << 3)) : 0;
+    // The edges incident to this segment's vertices are owned by the segment itself, by the
+    // segments of rows -x and -y, and (the z edge into lane 0) by the previous chunk.
+    const int64_t sm0 = px > 0 ? seg_index(g, px - 1, py, chunk) : -1;
+    const int64_t s0m = py > 0 ? seg_index(g, px, py - 1, chunk) : -1;
     Grad<S> acc = {S(0), S(0), S(0), S(0)};
-    if (__ballot(mask != 0) == 0) {  // no crossing edge touches this segment: zero gradients
-        store_grad(in, adj_sdf, adj_deform, px, py, pz, acc, adj.flags);
+    if (!seg_owns_crossing(seg, s) && !seg_owns_crossing(seg, sm0) && !seg_owns_crossing(seg, s0m) &&
+        !(chunk > 0 && seg_owns_crossing(seg, s - 1))) {
+        if (!(adj.flags & DISO_FLAG_ACCUMULATE)) store_grad(in, adj_sdf, adj_deform, px, py, pz, acc, adj.flags);
         return;
     }
-    w.template load_box<-1, 1, 1, 1, 0, 1>(in);
-    w.template load_box<-1, 1, 0, 1, 1, 1>(in);
-    w.template load_box<1, 1, -1, 1, 0, 1>(in);
-    w.template load_box<0, 1, -1, 1, 1, 1>(in);
-    w.template load_box<1, 1, 0, 1, -1, 1>(in);
-    w.template load_box<0, 1, 1, 1, -1, 1>(in);
-    const int mm0 = is_anchor(in, px - 1, py, pz) ? own_mask(smp[4], c, w(-1, 1, 0), w(-1, 0, 1), iso) : 0;
-    const int m0m = is_anchor(in, px, py - 1, pz) ? own_mask(smp[5], w(1, -1, 0), c, w(0, -1, 1), iso) : 0;
+    const int64_t vbase = seg[s].v;
+    // Crossing masks of the edges owned by the vertex and by its -x, -y, -z neighbours: from the
+    // classification words, or (RECOMP) from the samples of the vertex's neighbourhood.
+    int m00, mm0, m0m, mzm;
+    S c = S(0);
+    if constexpr (RECOMP) {
+        Win<S, -1, 3, -1, 3, -1, 3> w(in, px, py, pz, z0);
+        w.template load_box<0, 1, 0, 1, -1, 3>(in);
+        w.template load_box<-1, 1, 0, 1, 0, 2>(in);
+        w.template load_box<-1, 1, 1, 1, 0, 1>(in);
+        w.template load_box<1, 1, -1, 2, 0, 1>(in);
+        w.template load_box<0, 1, -1, 1, 0, 2>(in);
+        w.template load_box<0, 1, 1, 1, 0, 1>(in);
+        c = w(0, 0, 0);
+        m00 = is_anchor(in, px, py, pz) ? own_mask(c, w(1, 0, 0), w(0, 1, 0), w(0, 0, 1), iso) : 0;
+        mm0 = is_anchor(in, px - 1, py, pz) ? own_mask(w(-1, 0, 0), c, w(-1, 1, 0), w(-1, 0, 1), iso) : 0;
+        m0m = is_anchor(in, px, py - 1, pz) ? own_mask(w(0, -1, 0), w(1, -1, 0), c, w(0, -1, 1), iso) : 0;
+        mzm = is_anchor(in, px, py, pz - 1) ? crosses(w(0, 0, -1), c, iso) << 2 : 0;
+    } else {
+        const uint16_t *info = info_of(seg, g);
+        m00 = info_mask(info[s * kWave + lane]);
+        mm0 = sm0 >= 0 ? info_mask(info[sm0 * kWave + lane]) : 0;
+        m0m = s0m >= 0 ? info_mask(info[s0m * kWave + lane]) : 0;
+        mzm = pz > 0 ? info_mask(info[s * kWave + lane - 1]) : 0;
+    }
+    // slots 0-2: edges the vertex owns; 3-5: the x edge of -x, the y edge of -y, the z edge of -z
+    const int inc_m = (mm0 & 1) | (m0m & 2) | (mzm & 4);
+    const int mask = in_data(in, px, py, pz) ? (m00 | (inc_m << 3)) : 0;
     int t, total;
     const int32_t r00 = wave_excl_small(popc3(m00), &t);
-    const int32_t rm0 = (int32_t)(bm0 - vbase) + wave_excl_small(popc3(mm0), &t);
-    const int32_t r0m = (int32_t)(b0m - vbase) + wave_excl_small(popc3(m0m), &t);
+    const int32_t rm0 = (int32_t)((sm0 >= 0 ? seg[sm0].v : 0) - vbase) + wave_excl_small(popc3(mm0), &t);
+    const int32_t r0m = (int32_t)((s0m >= 0 ? seg[s0m].v : 0) - vbase) + wave_excl_small(popc3(m0m), &t);
     const int first = wave_excl_small(__popc(mask), &total);
 
     BwdLds<S> &L = s_lds[wave];
-#pragma unroll
-    for (int i = 0; i < 7; ++i) L.smp[i][lane] = smp[i];
     if (mask) {
+        L.c[lane] = RECOMP ? c : sample(in, px, py, pz);
         const V3<S> f = deform_at<S, DEF>(in, px, py, pz);
         L.fc[0][lane] = f.x;
         L.fc[1][lane] = f.y;
@@ -1214,11 +1163,11 @@ __global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase 
             const int it = L.item[k], src = it >> 3, slot = it & 7;
             const int axis = slot % 3, end = slot >= 3;
             const int x = px - (slot == 3), y = py - (slot == 4), z = z0 + src - (slot == 5);
-            const S d0 = slot < 3 ? L.smp[0][src] : L.smp[slot + 1][src];
-            const S d1 = slot < 3 ? L.smp[slot + 1][src] : L.smp[0][src];
-            const V3<S> fv = {L.fc[0][src], L.fc[1][src], L.fc[2][src]};
             // the edge's other endpoint
             const int ex = end ? x : x + (axis == 0), ey = end ? y : y + (axis == 1), ez = end ? z : z + (axis == 2);
+            const S dv = L.c[src], dox = sample(in, ex, ey, ez);
+            const S d0 = end ? dox : dv, d1 = end ? dv : dox;
+            const V3<S> fv = {L.fc[0][src], L.fc[1][src], L.fc[2][src]};
             const V3<S> fo = deform_at<S, DEF>(in, ex, ey, ez);
             L.res[lane] = edge_adjoint<S, DEF>(x, y, z, axis, end, d0, d1, end ? fo : fv, end ? fv : fo, iso,
                                                load_adj(adj, vbase + L.id[slot][src]));
@@ -1267,6 +1216,150 @@ __global__ __launch_bounds__(kBlock) void dmc_edge_grad_kernel(const void *__res
 }
 
 // ------------------------------------------------------------------------------------------
+// DiffDMC quad -> triangle split (diso/__init__.py:117-147)
+//
+// Each quad is cut along the diagonal whose worse triangle has the smaller largest corner cosine:
+// cut (0,1,3)+(1,2,3) ("first") when score1 < score2, else (0,1,2)+(0,2,3); the output holds all
+// first quads' pairs in quad order, then all the others'. split_score_kernel scores the quads of a
+// contiguous range per block and counts the first ones; split_emit_kernel gives each block the
+// prefix and the total of those counts (it reads all of them: the grid is ~4 blocks per CU, so a
+// few KB), ranks its quads with ballots and writes the pairs in place. The cosine arithmetic is
+// F.normalize's (x / max(|x|, 1e-12), |x| = sqrt((x*x + y*y) + z*z)) and a left-to-right dot.
+
+constexpr int kSplitBlock = 256;
+constexpr int kSplitWaves = kSplitBlock / kWave;
+constexpr int64_t kMaxSplitBlocks = 4096;
+
+template <typename S>
+__device__ __forceinline__ V3<S> unit(V3<S> v) {
+    const S n = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    const S d = n < S(1e-12) ? S(1e-12) : n;  // clamp_min(eps); keeps a NaN norm
+    return {v.x / d, v.y / d, v.z / d};
+}
+
+template <typename S>
+__device__ __forceinline__ S nan_max(S a, S b) {  // torch.max / np.maximum: NaN wins
+    return (a != a || a > b) ? a : b;
+}
+
+template <typename S>
+__device__ __forceinline__ S corner_cos(V3<S> o, V3<S> a, V3<S> b) {
+    const V3<S> u = unit(V3<S>{a.x - o.x, a.y - o.y, a.z - o.z});
+    const V3<S> w = unit(V3<S>{b.x - o.x, b.y - o.y, b.z - o.z});
+    return u.x * w.x + u.y * w.y + u.z * w.z;
+}
+
+// Largest corner cosine of triangle (p0, p1, p2) (diso/__init__.py:127-136).
+template <typename S>
+__device__ __forceinline__ S tri_worst(V3<S> p0, V3<S> p1, V3<S> p2) {
+    return nan_max(nan_max(corner_cos(p0, p1, p2), corner_cos(p1, p2, p0)), corner_cos(p2, p0, p1));
+}
+
+template <typename S>
+__device__ __forceinline__ V3<S> load_vert(const S *__restrict__ v, int64_t i) {
+    return {v[3 * i + 0], v[3 * i + 1], v[3 * i + 2]};
+}
+
+template <typename S>
+__global__ __launch_bounds__(kSplitBlock) void split_score_kernel(const S *__restrict__ verts, int64_t nv,
+                                                                  const void *__restrict__ quads, int64_t nq,
+                                                                  int flags, uint8_t *__restrict__ first,
+                                                                  uint32_t *__restrict__ bcount, int64_t nblocks) {
+    __shared__ uint32_t s_cnt[kSplitWaves];
+    const int tid = threadIdx.x, lane = tid % kWave, wave = wave_id();
+    const int64_t b = blockIdx.x, lo = b * nq / nblocks, hi = (b + 1) * nq / nblocks;
+    uint32_t cnt = 0;
+    for (int64_t q = lo + tid; q < hi; q += kSplitBlock) {
+        int64_t id[4];
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            id[k] = load_index(quads, 4 * q + k, flags);
+            ok &= id[k] >= 0 && id[k] < nv;
+        }
+        uint8_t f = 0;
+        if (ok) {
+            const V3<S> p0 = load_vert(verts, id[0]), p1 = load_vert(verts, id[1]), p2 = load_vert(verts, id[2]),
+                        p3 = load_vert(verts, id[3]);
+            const S s1 = nan_max(tri_worst(p0, p1, p3), tri_worst(p1, p2, p3));
+            const S s2 = nan_max(tri_worst(p0, p1, p2), tri_worst(p0, p2, p3));
+            f = s1 < s2;
+        }
+        first[q] = f;
+        cnt += f;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, o, kWave);
+    if (lane == 0) s_cnt[wave] = cnt;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kSplitWaves; ++w) t += s_cnt[w];
+        bcount[b] = t;
+    }
+}
+
+__global__ __launch_bounds__(kSplitBlock) void split_emit_kernel(const void *__restrict__ quads, int64_t nq,
+                                                                 int flags, const uint8_t *__restrict__ first,
+                                                                 const uint32_t *__restrict__ bcount,
+                                                                 int64_t nblocks, int64_t *__restrict__ tris) {
+    __shared__ int64_t s_pre[kSplitWaves], s_tot[kSplitWaves];
+    __shared__ uint32_t s_w[kSplitWaves];
+    const int tid = threadIdx.x, lane = tid % kWave, wave = wave_id();
+    const int64_t b = blockIdx.x, lo = b * nq / nblocks, hi = (b + 1) * nq / nblocks;
+    // First quads in the blocks before this one, and in all blocks.
+    int64_t pre = 0, tot = 0;
+    for (int64_t j = tid; j < nblocks; j += kSplitBlock) {
+        const int64_t c = bcount[j];
+        tot += c;
+        pre += j < b ? c : 0;
+    }
+    pre = wave_sum64(pre);
+    tot = wave_sum64(tot);
+    if (lane == 0) {
+        s_pre[wave] = pre;
+        s_tot[wave] = tot;
+    }
+    __syncthreads();
+    int64_t run = 0, n1 = 0;
+#pragma unroll
+    for (int w = 0; w < kSplitWaves; ++w) {
+        run += s_pre[w];
+        n1 += s_tot[w];
+    }
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int64_t base = lo; base < hi; base += kSplitBlock) {
+        const int64_t q = base + tid;
+        const bool f = q < hi && first[q];
+        const uint64_t bal = __ballot(f);
+        __syncthreads();  // s_w reuse
+        if (lane == 0) s_w[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        int64_t r1 = run + __popcll(bal & lt);
+        uint32_t blk = 0;
+#pragma unroll
+        for (int w = 0; w < kSplitWaves; ++w) {
+            r1 += w < wave ? s_w[w] : 0;
+            blk += s_w[w];
+        }
+        if (q < hi) {
+            const int64_t a = load_index(quads, 4 * q + 0, flags), c1 = load_index(quads, 4 * q + 1, flags),
+                          c2 = load_index(quads, 4 * q + 2, flags), c3 = load_index(quads, 4 * q + 3, flags);
+            // rows 2r, 2r+1 of the first half, or of the second half at rank q - r1 among the others
+            int64_t *t = tris + 6 * (f ? r1 : n1 + (q - r1));
+            if (f) {
+                t[0] = a, t[1] = c1, t[2] = c3;
+                t[3] = c1, t[4] = c2, t[5] = c3;
+            } else {
+                t[0] = a, t[1] = c1, t[2] = c2;
+                t[3] = a, t[4] = c2, t[5] = c3;
+            }
+        }
+        run += blk;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Host side
 
 thread_local char g_err[512] = "";
@@ -1303,12 +1396,14 @@ bool make_geo(int64_t X, int64_t Y, int64_t Z, int flags, Geo *g) {
     return (int64_t)g->g0 * g->g1 * g->g2 < (1ll << 36) && g->nseg < (1ll << 32);
 }
 
-int64_t seg_bytes_for(const Geo &g) { return (g.nseg + 2) * (int64_t)sizeof(SegBase); }
+// Segment bases (+ two totals entries), then one 16-bit classification word per lane.
+int64_t bases_bytes_for(const Geo &g) { return (g.nseg + 2) * (int64_t)sizeof(SegBase); }
+int64_t seg_bytes_for(const Geo &g) { return bases_bytes_for(g) + g.nseg * kWave * 2; }
 int64_t ctl_bytes_for(const Geo &) { return kCtlHeader + 2 * kMaxCountBlocks * (int64_t)sizeof(uint64_t); }
 
 // Number of count blocks: about kCountBlocksPerCU per CU of the current device, so that every
 // block is resident at once, and never more than the segments or the word buffer allow.
-int64_t count_blocks(const Geo &g) {
+int cu_count() {
     static int cus[64] = {0};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
@@ -1317,7 +1412,11 @@ int64_t count_blocks(const Geo &g) {
         if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
         cus[dev] = n;
     }
-    int64_t nb = (int64_t)cus[dev] * kCountBlocksPerCU;
+    return cus[dev];
+}
+
+int64_t count_blocks(const Geo &g) {
+    int64_t nb = (int64_t)cu_count() * kCountBlocksPerCU;
     const int64_t min_nb = (g.nseg + (1ll << 20) - 1) >> 20;  // <= 2^20 segments per block keeps the
     if (nb < min_nb) nb = min_nb;                            // aggregate fields from overflowing
     if (nb > kMaxCountBlocks) nb = kMaxCountBlocks;
@@ -1366,35 +1465,28 @@ Norm<S> norm_of(const Geo &g) {
 
 template <typename S>
 int launch_emit(int algo, const void *sdf, const void *deform, double iso, int flags, const Geo &g,
-                const void *seg, void *verts, int64_t vcap, void *faces, int64_t fcap, void *vinfo, void *scratch,
-                int64_t scratch_bytes, hipStream_t st) {
+                const void *seg, void *verts, int64_t vcap, void *faces, int64_t fcap, void *vinfo, hipStream_t st) {
     In<S> in = make_in<S>(sdf, deform, iso, g);
     const Norm<S> nm = norm_of<S>(g);
     const int64_t nb = row_blocks(g);
     const SegBase *sb = (const SegBase *)seg;
     const OutCfg out{flags, vcap, fcap};
+    const dim3 grid((unsigned)nb), block(kBlock);
     if (algo == DISO_MC) {
         if (deform)
-            hipLaunchKernelGGL((emit_mc_kernel<S, true, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
-                               (S *)verts, faces, out, nm, nb);
+            hipLaunchKernelGGL((emit_mc_kernel<S, true>), grid, block, 0, st, in, sb, (S *)verts, faces, out, nm,
+                               nb);
         else
-            hipLaunchKernelGGL((emit_mc_kernel<S, false, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
-                               (S *)verts, faces, out, nm, nb);
-        return check_hip(hipGetLastError(), "emit kernel launch");
+            hipLaunchKernelGGL((emit_mc_kernel<S, false>), grid, block, 0, st, in, sb, (S *)verts, faces, out, nm,
+                               nb);
+    } else {
+        if (deform)
+            hipLaunchKernelGGL((emit_dmc_kernel<S, true>), grid, block, 0, st, in, sb, (S *)verts, faces,
+                               (uint8_t *)vinfo, out, nm, nb);
+        else
+            hipLaunchKernelGGL((emit_dmc_kernel<S, false>), grid, block, 0, st, in, sb, (S *)verts, faces,
+                               (uint8_t *)vinfo, out, nm, nb);
     }
-    // DMC: MC edge vertices into scratch (no epilogue), then patch means and quads.
-    const int64_t mcap = scratch_bytes / (3 * (int64_t)sizeof(S));
-    const OutCfg mout{0, mcap, 0};
-    if (deform)
-        hipLaunchKernelGGL((emit_mc_kernel<S, true, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
-                           (S *)scratch, nullptr, mout, nm, nb);
-    else
-        hipLaunchKernelGGL((emit_mc_kernel<S, false, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb,
-                           (S *)scratch, nullptr, mout, nm, nb);
-    int rc = check_hip(hipGetLastError(), "edge vertex kernel launch");
-    if (rc) return rc;
-    hipLaunchKernelGGL((emit_dmc_kernel<S>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb, (const S *)scratch, mcap,
-                       (S *)verts, faces, (uint8_t *)vinfo, out, nm, nb);
     return check_hip(hipGetLastError(), "emit kernel launch");
 }
 
@@ -1424,11 +1516,43 @@ int launch_backward(int algo, const void *sdf, const void *deform, double iso, i
         a.n = nfaces;
         a.flags = flags & ~DISO_FLAG_NORMALIZE;  // applied by the edge gradient kernel
     }
-    if (deform)
-        hipLaunchKernelGGL((bwd_mc_kernel<S, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb, a, gs, gd, nb);
+    const dim3 grid((unsigned)nb), block(kBlock);
+    const bool rec = (flags & DISO_FLAG_RECOMPUTE) != 0;
+    if (deform && rec)
+        hipLaunchKernelGGL((bwd_mc_kernel<S, true, true>), grid, block, 0, st, in, sb, a, gs, gd, nb);
+    else if (deform)
+        hipLaunchKernelGGL((bwd_mc_kernel<S, true, false>), grid, block, 0, st, in, sb, a, gs, gd, nb);
+    else if (rec)
+        hipLaunchKernelGGL((bwd_mc_kernel<S, false, true>), grid, block, 0, st, in, sb, a, gs, gd, nb);
     else
-        hipLaunchKernelGGL((bwd_mc_kernel<S, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, in, sb, a, gs, gd, nb);
+        hipLaunchKernelGGL((bwd_mc_kernel<S, false, false>), grid, block, 0, st, in, sb, a, gs, gd, nb);
     return check_hip(hipGetLastError(), "backward kernel launch");
+}
+
+// Split: about 4 blocks per CU, each over a contiguous range of quads.
+int64_t split_blocks(int64_t nq) {
+    int64_t nb = (int64_t)cu_count() * 4;
+    const int64_t need = (nq + kSplitBlock - 1) / kSplitBlock;
+    if (nb > need) nb = need;
+    if (nb > kMaxSplitBlocks) nb = kMaxSplitBlocks;
+    return nb < 1 ? 1 : nb;
+}
+
+int64_t split_scratch_bytes(int64_t nq) { return kMaxSplitBlocks * (int64_t)sizeof(uint32_t) + nq; }
+
+template <typename S>
+int launch_split(const void *verts, int64_t nv, const void *quads, int64_t nq, int flags, int64_t *tris,
+                 void *scratch, hipStream_t st) {
+    const int64_t nb = split_blocks(nq);
+    uint32_t *bcount = static_cast<uint32_t *>(scratch);
+    uint8_t *first = static_cast<uint8_t *>(scratch) + kMaxSplitBlocks * sizeof(uint32_t);
+    hipLaunchKernelGGL((split_score_kernel<S>), dim3((unsigned)nb), dim3(kSplitBlock), 0, st, (const S *)verts, nv,
+                       quads, nq, flags, first, bcount, nb);
+    int rc = check_hip(hipGetLastError(), "split score kernel launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL(split_emit_kernel, dim3((unsigned)nb), dim3(kSplitBlock), 0, st, quads, nq, flags,
+                       (const uint8_t *)first, (const uint32_t *)bcount, nb, tris);
+    return check_hip(hipGetLastError(), "split emit kernel launch");
 }
 
 int check_common(int algo, int dtype, int64_t X, int64_t Y, int64_t Z, int flags, Geo *g) {
@@ -1471,7 +1595,7 @@ const char *diso_version(void) { return "diso_hip 0.1.0 gfx950"; }
 int64_t diso_seg_bytes(int64_t X, int64_t Y, int64_t Z, int flags) {
     Geo g;
     if (!make_geo(X, Y, Z, flags, &g)) return fail(DISO_E_INVALID, "unsupported grid dims");
-    return seg_bytes_for(g);
+    return (flags & DISO_FLAG_RECOMPUTE) ? bases_bytes_for(g) : seg_bytes_for(g);
 }
 
 int64_t diso_ctl_bytes(int64_t X, int64_t Y, int64_t Z, int flags) {
@@ -1513,7 +1637,7 @@ int diso_read_totals(const void *seg, int64_t seg_bytes, int64_t X, int64_t Y, i
     Geo g;
     if (!make_geo(X, Y, Z, flags, &g)) return fail(DISO_E_INVALID, "unsupported grid dims");
     if (!seg || !totals_host) return fail(DISO_E_INVALID, "null pointer argument");
-    if (seg_bytes < seg_bytes_for(g)) return fail(DISO_E_CAPACITY, "seg workspace too small");
+    if (seg_bytes < bases_bytes_for(g)) return fail(DISO_E_CAPACITY, "seg workspace too small");
     return read_totals(g, seg, totals_host, (hipStream_t)stream);
 }
 
@@ -1526,8 +1650,7 @@ int64_t diso_scratch_bytes(int algo, int dtype, int64_t n_crossing) {
 
 int diso_emit(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y, int64_t Z,
               double iso, int flags, const void *seg, int64_t seg_bytes, void *verts, int64_t vert_capacity,
-              void *faces, int64_t face_capacity, void *vinfo, void *scratch, int64_t scratch_bytes,
-              void *stream) {
+              void *faces, int64_t face_capacity, void *vinfo, void *stream) {
     g_err[0] = 0;
     Geo g;
     int rc = check_common(algo, dtype, X, Y, Z, flags, &g);
@@ -1536,16 +1659,12 @@ int diso_emit(int algo, int dtype, const void *sdf, const void *deform, int64_t 
     if (seg_bytes < seg_bytes_for(g)) return fail(DISO_E_CAPACITY, "seg workspace too small");
     if (vert_capacity < 0 || face_capacity < 0) return fail(DISO_E_INVALID, "negative capacity");
     if ((vert_capacity && !verts) || (face_capacity && !faces)) return fail(DISO_E_INVALID, "null output buffer");
-    if (algo == DISO_DMC) {
-        if (vert_capacity && !vinfo) return fail(DISO_E_INVALID, "DMC needs a vinfo buffer");
-        if (scratch_bytes < diso_scratch_bytes(algo, dtype, face_capacity) || (face_capacity && !scratch))
-            return fail(DISO_E_CAPACITY, "scratch buffer too small for %lld quads", (long long)face_capacity);
-    }
+    if (algo == DISO_DMC && vert_capacity && !vinfo) return fail(DISO_E_INVALID, "DMC needs a vinfo buffer");
     hipStream_t st = (hipStream_t)stream;
     return dtype == DISO_F32 ? launch_emit<float>(algo, sdf, deform, iso, flags, g, seg, verts, vert_capacity, faces,
-                                                  face_capacity, vinfo, scratch, scratch_bytes, st)
+                                                  face_capacity, vinfo, st)
                              : launch_emit<double>(algo, sdf, deform, iso, flags, g, seg, verts, vert_capacity, faces,
-                                                   face_capacity, vinfo, scratch, scratch_bytes, st);
+                                                   face_capacity, vinfo, st);
 }
 
 int diso_backward(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y, int64_t Z,
@@ -1559,7 +1678,8 @@ int diso_backward(int algo, int dtype, const void *sdf, const void *deform, int6
     if (!sdf || !seg || !adj_sdf) return fail(DISO_E_INVALID, "null pointer argument");
     if (deform && !adj_deform) return fail(DISO_E_INVALID, "adj_deform required when deform is given");
     if (n_verts < 0 || (n_verts && !adj_verts)) return fail(DISO_E_INVALID, "bad adj_verts");
-    if (seg_bytes < seg_bytes_for(g)) return fail(DISO_E_CAPACITY, "seg workspace too small");
+    if (seg_bytes < ((flags & DISO_FLAG_RECOMPUTE) ? bases_bytes_for(g) : seg_bytes_for(g)))
+        return fail(DISO_E_CAPACITY, "seg workspace too small");
     if (algo == DISO_DMC) {
         if (n_faces < 0 || (n_faces && (!faces || !vinfo))) return fail(DISO_E_INVALID, "DMC backward needs the quads and vinfo");
         if (scratch_bytes < diso_scratch_bytes(algo, dtype, n_faces) || (n_faces && !scratch))
@@ -1570,6 +1690,27 @@ int diso_backward(int algo, int dtype, const void *sdf, const void *deform, int6
                                                       adj_verts, n_verts, adj_sdf, adj_deform, scratch, st)
                              : launch_backward<double>(algo, sdf, deform, iso, flags, g, seg, faces, n_faces, vinfo,
                                                        adj_verts, n_verts, adj_sdf, adj_deform, scratch, st);
+}
+
+int64_t diso_split_scratch_bytes(int64_t n_quads) {
+    if (n_quads < 0) return fail(DISO_E_INVALID, "negative quad count");
+    return split_scratch_bytes(n_quads);
+}
+
+int diso_split(int dtype, const void *verts, int64_t n_verts, const void *quads, int64_t n_quads, int flags,
+               int64_t *tris, int64_t tri_capacity, void *scratch, int64_t scratch_bytes, void *stream) {
+    g_err[0] = 0;
+    if (dtype != DISO_F32 && dtype != DISO_F64) return fail(DISO_E_INVALID, "dtype must be DISO_F32 or DISO_F64");
+    if (n_quads < 0 || n_verts < 0) return fail(DISO_E_INVALID, "negative size");
+    if (n_quads == 0) return DISO_OK;
+    if (!verts || !quads || !tris || !scratch) return fail(DISO_E_INVALID, "null pointer argument");
+    if (tri_capacity < 2 * n_quads)
+        return fail(DISO_E_CAPACITY, "triangle buffer too small: %lld < %lld rows", (long long)tri_capacity,
+                    (long long)(2 * n_quads));
+    if (scratch_bytes < split_scratch_bytes(n_quads)) return fail(DISO_E_CAPACITY, "scratch buffer too small");
+    hipStream_t st = (hipStream_t)stream;
+    return dtype == DISO_F32 ? launch_split<float>(verts, n_verts, quads, n_quads, flags, tris, scratch, st)
+                             : launch_split<double>(verts, n_verts, quads, n_quads, flags, tris, scratch, st);
 }
 
 }  // extern "C"

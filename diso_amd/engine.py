@@ -88,7 +88,7 @@ class Extractor:
     def count(self, grid, iso, flags):
         dims = tuple(int(d) for d in grid.shape)
         L = _lib.lib()
-        nseg = L.diso_seg_bytes(*dims, flags)
+        nseg = L.diso_seg_bytes(*dims, flags & ~_lib.FLAG_RECOMPUTE)  # the full workspace
         if nseg < 0:
             _lib.check(nseg, "workspace sizing")
         seg = torch.empty(nseg, dtype=torch.uint8, device=grid.device)
@@ -114,13 +114,17 @@ class Extractor:
         verts = torch.empty((counted.n_verts, 3), dtype=self.dtype, device=grid.device)
         faces = torch.empty((counted.n_faces, k), dtype=idx, device=grid.device)
         vinfo = torch.empty(counted.n_verts if dmc else 0, dtype=torch.uint8, device=grid.device)
-        scratch, nscratch = self._scratch(counted, grid.device)
         rc = L.diso_emit(self.algo, self.code, _ptr(grid), _ptr(deform), *counted.dims, counted.iso, flags,
                          _ptr(counted.seg), counted.seg.numel(), _ptr(verts), counted.n_verts, _ptr(faces),
-                         counted.n_faces, _ptr(vinfo), _ptr(scratch), nscratch, _stream(grid.device))
+                         counted.n_faces, _ptr(vinfo), _stream(grid.device))
         _lib.check(rc, "emit pass")
         if dmc:
             counted.quads, counted.vinfo = faces, vinfo
+        if flags & _lib.FLAG_RECOMPUTE:
+            # The backward will re-derive the classification from the samples: keep only the
+            # segment bases and release the classification words.
+            nb = L.diso_seg_bytes(*counted.dims, flags)
+            counted.seg = counted.seg[:nb].clone()
         return verts, faces
 
     def backward(self, grid, deform, counted, flags, adj_verts, adj_grid, adj_deform):
@@ -144,3 +148,31 @@ class Extractor:
                              _ptr(counted.vinfo if dmc else None), _ptr(adj_verts), counted.n_verts,
                              _ptr(adj_grid), _ptr(adj_deform), _ptr(scratch), nscratch, _stream(grid.device))
         _lib.check(rc, "backward pass")
+
+
+def split_quads(verts, quads):
+    """DiffDMC quad -> triangle split on the GPU (diso_split; diso/__init__.py:117-147):
+    [Q,4] quads -> [2Q,3] int64 triangles, first-cut quads first."""
+    if verts.device.type != "cuda" or quads.device.type != "cuda":
+        raise RuntimeError("split_quads: verts and quads must be CUDA tensors")
+    if verts.dtype not in _DTYPES:
+        raise RuntimeError(f"split_quads: verts must be float32 or float64, got {verts.dtype}")
+    if quads.dtype not in (torch.int32, torch.int64) or quads.dim() != 2 or quads.shape[1] != 4:
+        raise ValueError("split_quads: quads must be an int32/int64 [Q, 4] tensor")
+    L = _lib.lib()
+    nq = int(quads.shape[0])
+    tris = torch.empty((2 * nq, 3), dtype=torch.int64, device=verts.device)
+    if nq == 0:
+        return tris
+    verts = verts.detach().contiguous()
+    quads = quads.contiguous()
+    nbytes = L.diso_split_scratch_bytes(nq)
+    if nbytes < 0:
+        _lib.check(nbytes, "split sizing")
+    with torch.cuda.device(verts.device):
+        scratch = torch.empty(nbytes, dtype=torch.uint8, device=verts.device)
+        flags = _lib.FLAG_INDEX64 if quads.dtype == torch.int64 else 0
+        rc = L.diso_split(_DTYPES[verts.dtype], _ptr(verts), int(verts.shape[0]), _ptr(quads), nq, flags, _ptr(tris),
+                          2 * nq, _ptr(scratch), nbytes, _stream(verts.device))
+    _lib.check(rc, "split")
+    return tris

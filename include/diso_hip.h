@@ -40,8 +40,12 @@ enum {
     DISO_FLAG_INDEX64 = 1 << 3,     /* faces as int64 (the wrapper's .long()); else int32        */
     DISO_FLAG_REFERENCE_COMPAT = 1 << 4, /* DMC backward reads each cell's first patch gradient
                                             for all its patches, as src/cudualmc.cu:990 does    */
-    DISO_FLAG_ACCUMULATE = 1 << 5   /* backward adds into the gradient buffers instead of
+    DISO_FLAG_ACCUMULATE = 1 << 5,  /* backward adds into the gradient buffers instead of
                                        overwriting them                                        */
+    DISO_FLAG_RECOMPUTE = 1 << 6    /* backward re-derives the crossing masks from the samples
+                                       instead of reading the classification words, so only
+                                       the bases prefix of seg (diso_seg_bytes with this flag)
+                                       must be kept from the forward                           */
 };
 
 enum {
@@ -73,9 +77,11 @@ typedef struct {
 } diso_totals_t;
 
 /* Size in bytes of the per-call segment workspace ("seg") for a grid of dims (X,Y,Z): the
- * per-segment vertex/face bases written by diso_count and read by diso_emit / diso_backward.
- * It needs no initialisation. Keep it (with the same inputs) to run diso_backward later; that
- * is the state the reference keeps inside CuMC between forward and backward (src/cumc.h:73-99). */
+ * per-segment vertex/face bases and a 16-bit classification word per grid position, written by
+ * diso_count and read by diso_emit / diso_backward. It needs no initialisation. Keep it (with the
+ * same inputs) to run diso_backward later; that is the state the reference keeps inside CuMC
+ * between forward and backward (src/cumc.h:73-99). With DISO_FLAG_RECOMPUTE in `flags` the size
+ * is that of the bases prefix alone, which is all a DISO_FLAG_RECOMPUTE backward reads. */
 int64_t diso_seg_bytes(int64_t X, int64_t Y, int64_t Z, int flags);
 
 /* Size in bytes of the control workspace ("ctl") that serves a grid of dims (X,Y,Z) or any
@@ -93,9 +99,9 @@ int diso_count(int algo, int dtype, const void *sdf, int64_t X, int64_t Y, int64
                int flags, void *seg, int64_t seg_bytes, void *ctl, int64_t ctl_bytes,
                diso_totals_t *totals_host, void *stream);
 
-/* Size in bytes of the scratch buffer diso_emit and diso_backward need for a counted grid
- * (n_crossing from diso_totals_t): none for MC; for DMC, the MC edge vertices (forward) or the
- * per-edge upstream gradients (backward), n_crossing * 3 scalars. */
+/* Size in bytes of the scratch buffer diso_backward needs for a counted grid (n_crossing from
+ * diso_totals_t): none for MC; for DMC, the per-edge upstream gradients, n_crossing * 3
+ * scalars. */
 int64_t diso_scratch_bytes(int algo, int dtype, int64_t n_crossing);
 
 /* Pass 2: write vertices [n_verts,3] and faces ([n_faces,3] MC triangles, [n_faces,4] DMC quads)
@@ -104,12 +110,11 @@ int64_t diso_scratch_bytes(int algo, int dtype, int64_t n_crossing);
  * (src/cudualmc.cu:907-955,1027-1056), plus the wrapper's shift / normalise / int64 epilogue
  * (diso/__init__.py:56-61,110-116) when the corresponding flags are set. Writes never exceed
  * vert_capacity / face_capacity (rows); if the counts from diso_count do not fit, nothing is
- * written. DMC only: `scratch` (diso_scratch_bytes) is overwritten, and `vinfo` receives one
- * byte per DMC vertex (edges in its patch | patch rank << 3) that diso_backward needs. */
+ * written. DMC only: `vinfo` receives one byte per DMC vertex (edges in its patch | patch
+ * rank << 3) that diso_backward needs. */
 int diso_emit(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y,
               int64_t Z, double iso, int flags, const void *seg, int64_t seg_bytes, void *verts,
-              int64_t vert_capacity, void *faces, int64_t face_capacity, void *vinfo,
-              void *scratch, int64_t scratch_bytes, void *stream);
+              int64_t vert_capacity, void *faces, int64_t face_capacity, void *vinfo, void *stream);
 
 /* Backward: gradients of sum(adj_verts * verts) with respect to sdf [X,Y,Z] and deform
  * [X,Y,Z,3] (adj_deform may be NULL). One thread per grid vertex gathers from its at most six
@@ -129,6 +134,18 @@ int diso_backward(int algo, int dtype, const void *sdf, const void *deform, int6
                   int64_t scratch_bytes, void *stream);
 
 /* Reads the totals of the last diso_count on `seg` (blocking copy on `stream`). */
+/* Size in bytes of the scratch buffer diso_split needs for n_quads quads. */
+int64_t diso_split_scratch_bytes(int64_t n_quads);
+
+/* DiffDMC quad -> triangle split (replaces diso/__init__.py:117-147): each quad [a,b,c,d] of
+ * `quads` ([n_quads,4], int64 with DISO_FLAG_INDEX64 in `flags`, else int32; indices into `verts`
+ * [n_verts,3] of `dtype`) is cut along the diagonal whose worse triangle has the smaller largest
+ * corner cosine (F.normalize semantics, eps 1e-12): (a,b,d)+(b,c,d) when its score is strictly
+ * smaller, else (a,b,c)+(a,c,d). `tris` ([2*n_quads,3] int64) receives the pairs of all quads
+ * taking the first cut, in quad order, then those of all the others. `scratch` is overwritten. */
+int diso_split(int dtype, const void *verts, int64_t n_verts, const void *quads, int64_t n_quads, int flags,
+               int64_t *tris, int64_t tri_capacity, void *scratch, int64_t scratch_bytes, void *stream);
+
 int diso_read_totals(const void *seg, int64_t seg_bytes, int64_t X, int64_t Y, int64_t Z,
                      int flags, diso_totals_t *totals_host, void *stream);
 

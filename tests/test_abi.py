@@ -34,9 +34,14 @@ def test_workspace_sizes():
     L = _lib.lib()
     n = 256
     seg = L.diso_seg_bytes(n, n, n, _lib.FLAG_VIRTUAL_PAD)
-    # one 16-byte base per (row, 64-wide z chunk) of the padded grid, plus two totals entries
-    assert seg == ((n + 2) * (n + 2) * ((n + 2 + 63) // 64) + 2) * 16
-    assert L.diso_seg_bytes(n, n, n, 0) == (n * n * ((n + 63) // 64) + 2) * 16
+    # one 16-byte base per (row, 64-wide z chunk) of the padded grid, plus two totals entries, then
+    # one 16-bit classification word per lane of every segment
+    nseg = (n + 2) * (n + 2) * ((n + 2 + 63) // 64)
+    assert seg == (nseg + 2) * 16 + nseg * 64 * 2
+    # a recompute-classification backward keeps only the bases
+    assert L.diso_seg_bytes(n, n, n, _lib.FLAG_VIRTUAL_PAD | _lib.FLAG_RECOMPUTE) == (nseg + 2) * 16
+    nseg = n * n * ((n + 63) // 64)
+    assert L.diso_seg_bytes(n, n, n, 0) == (nseg + 2) * 16 + nseg * 64 * 2
     assert L.diso_ctl_bytes(n, n, n, 0) > 0
     assert L.diso_seg_bytes(0, 4, 4, 0) < 0
     assert L.diso_seg_bytes(1 << 30, 1 << 30, 1 << 30, 0) < 0
@@ -55,16 +60,21 @@ def test_invalid_arguments_fail_before_any_device_work():
     fake = ctypes.c_void_p(0x1000)
     rc = L.diso_count(0, 0, fake, 4, 4, 4, 0.0, 0, fake, 16, fake, 1 << 20, ctypes.byref(tot), null)
     assert rc == -3 and b"seg" in L.diso_last_error()
-    rc = L.diso_emit(0, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 1 << 20, null, 5, null, 0, null, null, 0, null)
+    rc = L.diso_emit(0, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 1 << 20, null, 5, null, 0, null, null)
     assert rc == -1
-    rc = L.diso_emit(1, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 1 << 20, fake, 5, fake, 7, fake, fake, 16, null)
-    assert rc == -3 and b"scratch" in L.diso_last_error()
+    rc = L.diso_emit(1, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 1 << 20, fake, 5, fake, 7, null, null)
+    assert rc == -1 and b"vinfo" in L.diso_last_error()
+    rc = L.diso_emit(0, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 16, fake, 5, fake, 7, null, null)
+    assert rc == -3 and b"seg" in L.diso_last_error()
     rc = L.diso_backward(0, 0, fake, fake, 4, 4, 4, 0.0, 0, fake, 1 << 20, null, 0, null, fake, 3, fake, null,
                          null, 0, null)
     assert rc == -1 and b"adj_deform" in L.diso_last_error()
     rc = L.diso_backward(1, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 1 << 20, null, 9, null, fake, 3, fake, null,
                          null, 0, null)
     assert rc == -1 and b"quads" in L.diso_last_error()
+    rc = L.diso_backward(1, 0, fake, null, 4, 4, 4, 0.0, 0, fake, 1 << 20, fake, 9, fake, fake, 3, fake, null,
+                         fake, 16, null)
+    assert rc == -3 and b"scratch" in L.diso_last_error()
     assert L.diso_scratch_bytes(0, 0, 100) == 0 and L.diso_scratch_bytes(1, 1, 100) == 2400
     with pytest.raises(RuntimeError, match="count pass failed"):
         _lib.check(-1, "count pass")
@@ -86,17 +96,17 @@ def test_python_api_errors_match_the_reference():
     assert _C.CUDMCFloat().reference_compat and not _C.CUMCFloat().reference_compat
 
 
-def test_split_quads_matches_oracle_on_cpu_tensors():
-    """split_quads is plain torch: check it against the numpy restatement on random quads."""
-    from oracle import oracle as O
-
-    rng = np.random.default_rng(0)
-    verts = rng.standard_normal((50, 3)).astype(np.float32)
-    quads = rng.integers(0, 50, (200, 4))
-    want, s1, s2 = O.split_quads(verts, quads)
-    got = diso_amd.split_quads(torch.tensor(verts), torch.tensor(quads)).numpy()
-    tie = np.abs(s1 - s2) <= 1e-6
-    if not tie.any():
-        assert np.array_equal(got, want)
-    else:
-        assert got.shape == want.shape
+def test_split_arguments_checked_before_any_device_work():
+    L = _lib.lib()
+    null = ctypes.c_void_p(0)
+    fake = ctypes.c_void_p(0x1000)
+    assert L.diso_split(0, null, 0, null, 0, 0, null, 0, null, 0, null) == 0  # nothing to do
+    assert L.diso_split(3, fake, 4, fake, 1, 0, fake, 2, fake, 1 << 20, null) == -1
+    assert L.diso_split(0, fake, 4, null, 1, 0, fake, 2, fake, 1 << 20, null) == -1
+    rc = L.diso_split(0, fake, 4, fake, 5, 0, fake, 9, fake, 1 << 20, null)
+    assert rc == -3 and b"triangle buffer" in L.diso_last_error()
+    rc = L.diso_split(0, fake, 4, fake, 5, 0, fake, 10, fake, 8, null)
+    assert rc == -3 and b"scratch" in L.diso_last_error()
+    assert L.diso_split_scratch_bytes(1000) >= 1000 and L.diso_split_scratch_bytes(-1) < 0
+    with pytest.raises(RuntimeError, match="CUDA"):
+        diso_amd.split_quads(torch.zeros(4, 3), torch.zeros(1, 4, dtype=torch.int64))

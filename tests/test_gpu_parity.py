@@ -171,3 +171,51 @@ def test_deterministic_and_repeatable():
     for a, b in zip(outs[:2], outs[2:]):
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("algo", [O.MC, O.DMC])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("dims,kind", [((37, 29, 131), "noise"), ((24, 24, 24), "noisy_box")])
+def test_recompute_backward_is_identical(algo, dtype, dims, kind):
+    """The recompute-classification backward (only the segment bases kept from the forward)
+    gives bit-identical gradients to the stored-classification backward, and both match the
+    oracle."""
+    sdf_np, def_np = inputs(dims, kind, 7, dtype, True)
+    print(f"case recompute algo={algo} dtype={np.dtype(dtype).name} dims={dims} kind={kind}", flush=True)
+    ups = None
+    grads = []
+    for rec in (False, True):
+        mod = (diso_amd.DiffMC(TDT[dtype], recompute_backward=rec) if algo == O.MC
+               else diso_amd.DiffDMC(TDT[dtype], recompute_backward=rec))
+        sdf = torch.tensor(sdf_np, device=DEV, requires_grad=True)
+        deform = torch.tensor(def_np, device=DEV, requires_grad=True)
+        kw = {"return_quads": True} if algo == O.DMC else {}
+        v, f = mod(sdf, deform, **kw)
+        if ups is None:
+            ups = torch.tensor(np.random.default_rng(3).standard_normal(tuple(v.shape)), dtype=v.dtype, device=DEV)
+        torch.autograd.backward(v, grad_tensors=ups)
+        grads.append((sdf.grad.cpu().numpy(), deform.grad.cpu().numpy()))
+    for a, b in zip(grads[0], grads[1]):
+        assert np.array_equal(a, b)
+    ref = O.Extractor(algo, dtype)
+    ref.forward(sdf_np, def_np, 0.0, True, return_quads=True)
+    gs, gd = ref.backward(ups.cpu().numpy())
+    grad_close(grads[1][0], gs, TOL[dtype])
+    grad_close(grads[1][1], gd, TOL[dtype])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("nq,idx", [(0, torch.int64), (1, torch.int64), (3000, torch.int32), (200_003, torch.int64)])
+def test_split_kernel_matches_oracle(dtype, nq, idx):
+    """The HIP split on random quads (including degenerate ones with repeated corners) against
+    the oracle's restatement of diso/__init__.py:117-147."""
+    print(f"case split dtype={np.dtype(dtype).name} nq={nq} idx={idx}", flush=True)
+    rng = np.random.default_rng(nq)
+    nv = max(4, nq // 2)
+    verts = rng.standard_normal((nv, 3)).astype(dtype)
+    quads = rng.integers(0, nv, (nq, 4))
+    quads[: nq // 50, 2] = quads[: nq // 50, 0]  # zero-length diagonals / sides
+    want, s1, s2 = O.split_quads(verts, quads)
+    got = diso_amd.split_quads(torch.tensor(verts, device=DEV), torch.tensor(quads, dtype=idx, device=DEV))
+    assert got.dtype == torch.int64 and tuple(got.shape) == (2 * nq, 3)
+    assert np.array_equal(got.cpu().numpy(), want)
