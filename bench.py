@@ -41,6 +41,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=3)
     p.add_argument("--copy-mib", type=int, default=256)
+    p.add_argument("--batch-count", type=int, default=64, help="grids of the uneven batch (0: skip it)")
+    p.add_argument("--batch-reps", type=int, default=5)
     return p.parse_args()
 
 
@@ -160,6 +162,68 @@ def profiled_traffic(phase, n):
             kv = dict(x.split("=", 1) for x in f[2:] if "=" in x)
             return int(kv["total_bytes"])
     return None
+
+
+def batched_vs_plain(dev, count, seed, reps):
+    """The uneven batch: `count` grids, N uniform over {32,...,256}, mixed families, deform,
+    MC/DMC alternating, DMC returning quads; forward + backward of all grids per pass, timed as
+    one BatchExtractor call and as a Python loop of per-grid DiffMC / DiffDMC calls."""
+    import torch
+
+    import diso_amd
+    from diso_amd import workloads
+
+    data = workloads.uneven_batch_torch(count, seed=seed, device=dev)
+    voxels = sum(n ** 3 for *_, n in data)
+    algos = [a for _, _, a, _, _ in data]
+    mods = {"mc": diso_amd.DiffMC(torch.float32), "dmc": diso_amd.DiffDMC(torch.float32)}
+    bex = diso_amd.BatchExtractor(torch.float32)
+    gen = torch.Generator(device=dev).manual_seed(seed + 7)
+    ups = None
+
+    def leaves():
+        return ([s.detach().requires_grad_(True) for s, *_ in data],
+                [d.detach().requires_grad_(True) for _, d, *_ in data])
+
+    def plain():
+        sdfs, defs = leaves()
+        outs = []
+        for i in range(count):
+            kw = {"return_quads": True} if algos[i] == "dmc" else {}
+            outs.append(mods[algos[i]](sdfs[i], defs[i], 0.0, **kw))
+        return outs
+
+    def batched():
+        sdfs, defs = leaves()
+        return bex(sdfs, algos, defs, return_quads=True)
+
+    def step(fn):
+        outs = fn()
+        live = [(v, u) for (v, _), u in zip(outs, ups) if v.shape[0]]
+        torch.autograd.backward([v for v, _ in live], grad_tensors=[u for _, u in live])
+
+    with torch.no_grad():
+        ups = [torch.randn(tuple(v.shape), generator=gen, device=dev) for v, _ in plain()]
+    res = {}
+    for name, fn in (("plain_loop", plain), ("batched", batched)):
+        for _ in range(2):
+            step(fn)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            step(fn)
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / reps
+    return {
+        "workload": f"{count} grids, N uniform over 32..256 (total {voxels} voxels), sphere / rounded box / "
+                    "blobs / noise, deform U(-0.3,0.3), MC and DMC alternating, fwd+bwd",
+        "voxels": int(voxels),
+        "batched_gvoxel_s": round(voxels / res["batched"] / 1e9, 3),
+        "plain_loop_gvoxel_s": round(voxels / res["plain_loop"] / 1e9, 3),
+        "batched_ms": round(res["batched"] * 1e3, 3),
+        "plain_loop_ms": round(res["plain_loop"] * 1e3, 3),
+        "speedup": round(res["plain_loop"] / res["batched"], 3),
+    }
 
 
 def cpu_baseline(sdf_np, deform_np, n, seed, reps):
@@ -284,8 +348,13 @@ def main():
         "kernels": kern,
         "copy_calibration": copy,
     }
+    if args.batch_count > 0:
+        del sets, mods
+        torch.cuda.empty_cache()
+        out["batched"] = batched_vs_plain(dev, args.batch_count, args.seed, args.batch_reps)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sets[0][3], sets[0][4], n, args.seed, args.cpu_reps)
+        sdf_np, def_np, _ = workloads.noisy_round_box(n, seed=args.seed)
+        out["cpu_baseline"] = cpu_baseline(sdf_np, def_np, n, args.seed, args.cpu_reps)
     print(json.dumps(out))
 
 

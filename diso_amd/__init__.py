@@ -17,9 +17,10 @@ import torch
 from torch import nn
 
 from . import _lib
+from .batch import BatchExtractor
 from .engine import Extractor, split_quads
 
-__all__ = ["DiffMC", "DiffDMC", "split_quads"]
+__all__ = ["DiffMC", "DiffDMC", "BatchExtractor", "split_quads"]
 __version__ = "0.1.0"
 
 

@@ -33,10 +33,14 @@ FLAG_RECOMPUTE = 1 << 6
 TOTAL_HAS_BELOW = 1 << 0
 TOTAL_HAS_ABOVE = 1 << 1
 TOTAL_HAS_NAN = 1 << 2
+PARAM_RUN = 0
+PARAM_COUNT_BLOCKS_PER_CU = 1
 
 EXPORTS = [
     "diso_seg_bytes", "diso_ctl_bytes", "diso_scratch_bytes", "diso_count", "diso_emit", "diso_backward",
     "diso_read_totals", "diso_last_error", "diso_version", "diso_split_scratch_bytes", "diso_split",
+    "diso_batch_seg_bytes", "diso_batch_ctl_bytes", "diso_batch_count", "diso_batch_emit", "diso_batch_scratch_bytes",
+    "diso_batch_backward", "diso_set_param", "diso_get_param",
 ]
 
 
@@ -90,6 +94,23 @@ def lib():
         L.diso_split_scratch_bytes.argtypes = [i64]
         L.diso_split.restype = i32
         L.diso_split.argtypes = [i32, vp, i64, vp, i64, i32, vp, i64, vp, i64, vp]
+        # batch entry points take the diso_grid_t array by address (diso_amd.batch.GridT)
+        L.diso_batch_seg_bytes.restype = i64
+        L.diso_batch_seg_bytes.argtypes = [vp, i32]
+        L.diso_batch_ctl_bytes.restype = i64
+        L.diso_batch_ctl_bytes.argtypes = [i32]
+        L.diso_batch_count.restype = i32
+        L.diso_batch_count.argtypes = [i32, vp, i32, vp, i64, vp, i64, vp]
+        L.diso_batch_emit.restype = i32
+        L.diso_batch_emit.argtypes = [i32, vp, i32, vp, i64, vp]
+        L.diso_batch_scratch_bytes.restype = i64
+        L.diso_batch_scratch_bytes.argtypes = [i32, vp, i32]
+        L.diso_batch_backward.restype = i32
+        L.diso_batch_backward.argtypes = [i32, vp, i32, vp, i64, vp, i64, vp]
+        L.diso_set_param.restype = i32
+        L.diso_set_param.argtypes = [i32, i32]
+        L.diso_get_param.restype = i32
+        L.diso_get_param.argtypes = [i32]
         L.diso_read_totals.restype = i32
         L.diso_read_totals.argtypes = [vp, i64, i64, i64, i64, i32, ctypes.POINTER(Totals), vp]
         L.diso_last_error.restype = ctypes.c_char_p
@@ -97,6 +118,9 @@ def lib():
         L.diso_version.restype = ctypes.c_char_p
         L.diso_version.argtypes = []
         _lib = L
+        from . import autotune
+
+        autotune.load()  # launch parameters saved for this device, if any
     return _lib
 
 

@@ -1,0 +1,209 @@
+"""Batched extraction: many grids of uneven size, Marching Cubes and Dual Marching Cubes mixed,
+forward and backward with one launch per pass (the C ABI's diso_batch_*).
+
+    ex = BatchExtractor(torch.float32)
+    outs = ex([sdf0, sdf1, ...], ["mc", "dmc", ...], deforms=[d0, None, ...], return_quads=True)
+    loss = sum(f(v) for v, _ in outs); loss.backward()
+
+Each (verts, faces) pair equals what DiffMC / DiffDMC (diso_amd/__init__.py, the reference's
+diso/__init__.py:9-147) return for that grid alone, bit for bit, gradients included: the same
+kernels run, only the launches are shared. A batch costs one host synchronisation in the forward
+(to size the outputs) plus one short descriptor upload per pass.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from .engine import _DTYPES, _ptr, _stream, split_quads
+
+__all__ = ["BatchExtractor"]
+
+
+class GridT(ctypes.Structure):
+    """diso_grid_t (include/diso_hip.h)."""
+
+    _fields_ = [("algo", ctypes.c_int32), ("flags", ctypes.c_int32), ("X", ctypes.c_int64), ("Y", ctypes.c_int64),
+                ("Z", ctypes.c_int64), ("iso", ctypes.c_double), ("sdf", ctypes.c_void_p),
+                ("deform", ctypes.c_void_p), ("verts", ctypes.c_void_p), ("faces", ctypes.c_void_p),
+                ("vinfo", ctypes.c_void_p), ("vert_capacity", ctypes.c_int64), ("face_capacity", ctypes.c_int64),
+                ("adj_verts", ctypes.c_void_p), ("adj_sdf", ctypes.c_void_p), ("adj_deform", ctypes.c_void_p),
+                ("totals", _lib.Totals)]
+
+
+_ALGOS = {"mc": _lib.MC, "dmc": _lib.DMC, _lib.MC: _lib.MC, _lib.DMC: _lib.DMC}
+
+
+class _State:
+    """Per-call state shared by forward and backward: descriptors, workspace, saved outputs."""
+
+    def __init__(self, ex, n):
+        self.ex = ex
+        self.n = n
+        self.desc = (GridT * n)()
+        self.seg = None
+        self.quads = [None] * n
+        self.vinfo = [None] * n
+        self.empty = [False] * n
+        self.has_deform = [False] * n
+
+
+class _BatchExtract(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, st, *tensors):
+        n = st.n
+        sdfs, deforms = tensors[:n], tensors[n:]
+        L = _lib.lib()
+        code = st.ex.code
+        dev = sdfs[0].device
+        stream = _stream(dev)
+        d = st.desc
+        nbytes = L.diso_batch_seg_bytes(ctypes.addressof(d), n)
+        _lib.check(0 if nbytes >= 0 else nbytes, "batch workspace sizing")
+        st.seg = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        ctl = st.ex.ctl_for(dev, n)
+        rc = L.diso_batch_count(code, ctypes.addressof(d), n, _ptr(st.seg), nbytes, _ptr(ctl), ctl.numel(), stream)
+        _lib.check(rc, "batch count pass")
+        both = _lib.TOTAL_HAS_BELOW | _lib.TOTAL_HAS_ABOVE
+        verts, faces = [], []
+        for i in range(n):
+            t = d[i].totals
+            if t.flags & _lib.TOTAL_HAS_NAN:
+                raise RuntimeError(f"diso_amd: sdf grid {i} of the batch contains NaN values")
+            dmc = d[i].algo == _lib.DMC
+            st.empty[i] = (t.flags & both) != both
+            if st.empty[i]:
+                # diso/__init__.py:49-50,103-104: the grid's early-out
+                d[i].totals.n_verts = 0
+                d[i].totals.n_faces = 0
+                verts.append(torch.zeros((0, 3), dtype=st.ex.dtype, device=dev))
+                faces.append(torch.zeros((0, 4 if dmc else 3), dtype=torch.int32, device=dev))
+                continue
+            v = torch.empty((t.n_verts, 3), dtype=st.ex.dtype, device=dev)
+            f = torch.empty((t.n_faces, 4 if dmc else 3), dtype=torch.int64, device=dev)
+            vi = torch.empty(t.n_verts if dmc else 0, dtype=torch.uint8, device=dev)
+            d[i].verts, d[i].faces, d[i].vinfo = v.data_ptr(), f.data_ptr(), vi.data_ptr() if dmc else None
+            d[i].vert_capacity, d[i].face_capacity = t.n_verts, t.n_faces
+            if dmc:
+                st.quads[i], st.vinfo[i] = f, vi
+            verts.append(v)
+            faces.append(f)
+        rc = L.diso_batch_emit(code, ctypes.addressof(d), n, _ptr(st.seg), nbytes, stream)
+        _lib.check(rc, "batch emit pass")
+        ctx.st = st
+        ctx.save_for_backward(*sdfs, *[x if x is not None else None for x in deforms])
+        ctx.mark_non_differentiable(*faces)
+        return (*verts, *faces)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        st = ctx.st
+        n = st.n
+        saved = ctx.saved_tensors
+        sdfs, deforms = saved[:n], saved[n:]
+        L = _lib.lib()
+        d = st.desc
+        g_sdf, g_def = [None] * n, [None] * n
+        keep = []
+        for i in range(n):
+            gv = grads[i]
+            d[i].adj_verts = d[i].adj_sdf = d[i].adj_deform = None
+            if gv is None or st.empty[i]:
+                continue
+            gv = gv.contiguous()
+            if gv.dtype != st.ex.dtype:
+                raise RuntimeError("adj_verts type must match the extractor's dtype")
+            g_sdf[i] = torch.empty_like(sdfs[i])
+            if deforms[i] is not None:
+                g_def[i] = torch.empty_like(deforms[i])
+            keep.append(gv)
+            d[i].adj_verts = gv.data_ptr()
+            d[i].adj_sdf = g_sdf[i].data_ptr()
+            d[i].adj_deform = None if g_def[i] is None else g_def[i].data_ptr()
+        if keep:
+            dev = sdfs[0].device
+            nsc = L.diso_batch_scratch_bytes(st.ex.code, ctypes.addressof(d), n)
+            _lib.check(0 if nsc >= 0 else nsc, "batch scratch sizing")
+            scratch = torch.empty(max(nsc, 1), dtype=torch.uint8, device=dev)
+            with torch.cuda.device(dev):
+                rc = L.diso_batch_backward(st.ex.code, ctypes.addressof(d), n, _ptr(st.seg), st.seg.numel(), _ptr(scratch), nsc,
+                                           _stream(dev))
+            _lib.check(rc, "batch backward pass")
+        return (None, *g_sdf, *g_def)
+
+
+class BatchExtractor:
+    """Extracts a list of grids in one batch. `algos` holds "mc" / "dmc" per grid; `deforms` is
+    None or a list with a [X,Y,Z,3] tensor or None per grid. All grids share the dtype, the
+    device, `isovalue` and `normalize`; DMC grids return quads when `return_quads`, else the
+    reference's triangle split."""
+
+    def __init__(self, dtype=torch.float32, reference_compat=False, recompute_backward=False):
+        if dtype not in _DTYPES:
+            raise ValueError(f"diso_amd: dtype must be torch.float32 or torch.float64, got {dtype}")
+        self.dtype = dtype
+        self.code = _DTYPES[dtype]
+        self.reference_compat = reference_compat
+        self.recompute_backward = recompute_backward
+        self._ctl = {}
+        _lib.lib()
+
+    def ctl_for(self, device, n):
+        need = _lib.lib().diso_batch_ctl_bytes(n)
+        _lib.check(0 if need >= 0 else need, "batch control sizing")
+        ctl = self._ctl.get(device.index)
+        if ctl is None or ctl.numel() < need:
+            ctl = torch.zeros(need * 5 // 4 + 256, dtype=torch.uint8, device=device)  # zero once, then reused
+            self._ctl[device.index] = ctl
+        return ctl
+
+    def __call__(self, grids, algos, deforms=None, isovalue=0.0, normalize=True, return_quads=False):
+        n = len(grids)
+        if n == 0:
+            return []
+        if len(algos) != n:
+            raise ValueError("one algorithm per grid")
+        deforms = [None] * n if deforms is None else list(deforms)
+        if len(deforms) != n:
+            raise ValueError("one deform (or None) per grid")
+        dev = grids[0].device
+        st = _State(self, n)
+        sdfs, defs = [], []
+        for i in range(n):
+            g, df = grids[i], deforms[i]
+            if not isinstance(g, torch.Tensor) or g.dim() != 3:
+                raise ValueError(f"grid {i} must be a [X, Y, Z] tensor")
+            if g.device.type != "cuda" or g.device != dev:
+                raise RuntimeError(f"grid {i} must be a CUDA tensor on {dev}")
+            if g.dtype != self.dtype:
+                raise RuntimeError(f"grid {i} type must match the extractor's dtype")
+            if df is not None and (tuple(df.shape) != tuple(g.shape) + (3,) or df.dtype != self.dtype
+                                   or df.device != dev):
+                raise ValueError(f"deform {i} must be a {list(g.shape) + [3]} tensor of the grid's dtype and device")
+            algo = _ALGOS.get(algos[i])
+            if algo is None:
+                raise ValueError(f"algorithm {algos[i]!r}: use 'mc' or 'dmc'")
+            g = g.contiguous()
+            df = None if df is None else df.contiguous()
+            sdfs.append(g)
+            defs.append(df)
+            flags = _lib.FLAG_VIRTUAL_PAD | _lib.FLAG_SHIFT | _lib.FLAG_INDEX64
+            if normalize:
+                flags |= _lib.FLAG_NORMALIZE
+            if self.reference_compat and algo == _lib.DMC:
+                flags |= _lib.FLAG_REFERENCE_COMPAT
+            if self.recompute_backward:
+                flags |= _lib.FLAG_RECOMPUTE
+            dsc = st.desc[i]
+            dsc.algo, dsc.flags = algo, flags
+            dsc.X, dsc.Y, dsc.Z = (int(x) for x in g.shape)
+            dsc.iso = float(isovalue)
+            dsc.sdf = g.data_ptr()
+            dsc.deform = None if df is None else df.data_ptr()
+        with torch.cuda.device(dev):
+            outs = _BatchExtract.apply(st, *sdfs, *defs)
+            verts, faces = list(outs[:n]), list(outs[n:])
+            for i in range(n):
+                if st.desc[i].algo == _lib.DMC and not return_quads and not st.empty[i]:
+                    faces[i] = split_quads(verts[i], faces[i])
+        return list(zip(verts, faces))

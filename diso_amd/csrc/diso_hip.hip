@@ -29,6 +29,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <vector>
+
 #include "diso_hip.h"
 #include "diso_tables.h"
 
@@ -246,6 +248,16 @@ __device__ __forceinline__ const uint16_t *info_of(const SegBase *seg, const Geo
     return reinterpret_cast<const uint16_t *>(seg + g.nseg + 2);
 }
 
+// A grid's segments as the emission and backward passes see them: its bases (seg[s], s from 0;
+// seg[nseg] is the base after its last segment), its classification words, and the base values
+// of its first segment (v0, f0), which are subtracted to index its own outputs. A grid extracted
+// alone has v0 = f0 = 0; in a batch the scan runs over all grids' segments in order.
+struct SegCtx {
+    const SegBase *seg;
+    const uint16_t *info;
+    int64_t v0, f0;
+};
+
 // ------------------------------------------------------------------------------------------
 // Register windows and edge geometry shared by the emission and backward kernels
 
@@ -302,6 +314,25 @@ __device__ __forceinline__ SegPos seg_pos(const Geo &g, int64_t s) {
     const uint32_t su = (uint32_t)s, nc = (uint32_t)g.nchunk, g1 = (uint32_t)g.g1;
     const uint32_t row = su / nc;
     return SegPos{(int)(row / g1), (int)(row % g1), (int)(su % nc)};
+}
+
+// The position of segment lo + i, read from lane i of a lane-parallel seg_pos (i uniform).
+__device__ __forceinline__ SegPos pos_of_lane(const SegPos &lp, int i) {
+    return SegPos{__builtin_amdgcn_readlane(lp.px, i), __builtin_amdgcn_readlane(lp.py, i),
+                  __builtin_amdgcn_readlane(lp.chunk, i)};
+}
+
+// A wave-uniform struct read from memory, kept in scalar registers.
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "dword-sized");
+    int w[sizeof(T) / 4];
+    __builtin_memcpy(w, p, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
 }
 
 // Wave index within the block, as a scalar (wave-uniform) value.
@@ -422,9 +453,10 @@ __device__ __forceinline__ uint64_t pack_agg(int64_t v, int64_t f, uint32_t fl) 
 template <typename S>
 using CornerWin = Win<S, 0, 2, 0, 2, 0, 2>;
 
-template <typename S, int ALGO>
-__device__ __forceinline__ void count_position(const In<S> &in, const uint64_t *cases, const CornerWin<S> &w,
-                                               int *nv, int *nf, uint32_t *fl, uint32_t *info) {
+template <typename S>
+__device__ __forceinline__ void count_position(const In<S> &in, int ALGO, const uint64_t *cases,
+                                               const CornerWin<S> &w, int *nv, int *nf, uint32_t *fl,
+                                               uint32_t *info) {
     const int px = w.px, py = w.py, pz = w.pz;
     const S v = w(0, 0, 0);
     *fl = in_data(in, px, py, pz) ? ((v < in.iso ? DISO_TOTAL_HAS_BELOW : 0u) |
@@ -468,10 +500,25 @@ __device__ __forceinline__ int64_t wave_incl64(int64_t v) {
     return v;
 }
 
-template <typename S, int ALGO>
-__global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *seg, uint8_t *ctl,
-                                                            int64_t nblocks) {
-    __shared__ uint64_t s_case[256];
+// What a count launch classifies: one grid (`in`, algorithm ALGO), or (BATCH) the concatenated
+// segments of several grids, each with its own descriptor; then per-grid sample flags are
+// or-ed into gflags[grid] (zero on entry) and the scan runs over all segments in order.
+template <typename S>
+struct BatchGrid;
+
+template <typename S>
+struct CountSrc {
+    In<S> in;
+    uint32_t *gflags;
+    int ngrids;
+    int64_t nseg;  // segments of the launch
+};
+
+template <typename S, int ALGO, bool BATCH>
+__global__ __launch_bounds__(kCountBlock, kCountBlocksPerCU * kCountWaves / 4) void count_kernel(
+    CountSrc<S> src, const BatchGrid<S> *__restrict__ grids, SegBase *seg, uint16_t *info, uint8_t *ctl,
+    int64_t nblocks) {
+    __shared__ uint64_t s_case[BATCH ? 2 : 1][256];
     __shared__ int64_t s_v[kCountWaves], s_f[kCountWaves];
     __shared__ uint32_t s_fl[kCountWaves];
     __shared__ uint32_t s_parity, s_dirty_other;
@@ -479,11 +526,18 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *s
     CtlHeader *hdr = reinterpret_cast<CtlHeader *>(ctl);
     uint64_t *words = reinterpret_cast<uint64_t *>(ctl + kCtlHeader);
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
-    const Geo &g = in.geo;
+    const int64_t nseg = src.nseg;
     const int64_t b = blockIdx.x;
-    const int64_t lo = b * g.nseg / nblocks, hi = (b + 1) * g.nseg / nblocks;
+    const int64_t lo = b * nseg / nblocks, hi = (b + 1) * nseg / nblocks;
 
-    for (int i = tid; i < 256; i += kCountBlock) s_case[i] = ALGO == DISO_MC ? kMcCase[i] : kDmcCase[i];
+    for (int i = tid; i < 256; i += kCountBlock) {
+        if (BATCH) {
+            s_case[0][i] = kMcCase[i];
+            s_case[BATCH ? 1 : 0][i] = kDmcCase[i];
+        } else {
+            s_case[0][i] = ALGO == DISO_MC ? kMcCase[i] : kDmcCase[i];
+        }
+    }
     if (tid == 0) {
         s_parity = __hip_atomic_load(&hdr->parity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_dirty_other =
@@ -493,37 +547,62 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *s
     const uint32_t parity = s_parity;
     uint64_t *cur = words + (int64_t)parity * kMaxCountBlocks;
 
-    // 1. Count: wave w takes segments lo+w, lo+w+8, ...; lane 0 records each segment's counts,
-    // every lane its classification word.
-    uint16_t *info = const_cast<uint16_t *>(info_of(seg, g));
+    // 1. Count: wave w takes the w-th eighth of the block's segments, in order (its position
+    // advances without divisions); lane 0 records each segment's counts, every lane its
+    // classification word.
     int64_t wv = 0, wf = 0;
     uint32_t wfl = 0;
-    // Two segments per iteration, both windows loaded before either is classified, so that two
-    // memory round trips overlap.
-    for (int64_t s0 = lo + wave; s0 < hi; s0 += 2 * kCountWaves) {  // wave-uniform
-        const int64_t s1 = s0 + kCountWaves < hi ? s0 + kCountWaves : s0;
-        const SegPos a = seg_pos(g, s0), b = seg_pos(g, s1);
-        CornerWin<S> wa(in, a.px, a.py, a.chunk * kWave + lane, a.chunk * kWave);
-        CornerWin<S> wb(in, b.px, b.py, b.chunk * kWave + lane, b.chunk * kWave);
-        wa.template load_box<0, 2, 0, 2, 0, 2>(in);
-        wb.template load_box<0, 2, 0, 2, 0, 2>(in);
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int64_t s = h ? s1 : s0;
-            if (h && s1 == s0) break;
-            int nv, nf;
-            uint32_t fl, iw;
-            count_position<S, ALGO>(in, s_case, h ? wb : wa, &nv, &nf, &fl, &iw);
-            info[s * kWave + lane] = (uint16_t)iw;
-            const int tv = wave_total_small(nv), tf = wave_total_small(nf);
-            if (lane == 0) {
-                seg[s].v = tv;
-                seg[s].f = tf;
+    const int64_t wlo = lo + (hi - lo) * wave / kCountWaves, whi = lo + (hi - lo) * (wave + 1) / kCountWaves;
+    int gi = 0;        // BATCH: grid of the current segment (segments only increase)
+    uint32_t gfl = 0;  // BATCH: its sample flags so far
+    In<S> in = src.in;
+    if constexpr (BATCH) {
+        while (gi + 1 < src.ngrids && wlo >= grids[gi + 1].seg_first) ++gi;
+        in = load_uniform(&grids[gi].in);
+    }
+    SegPos sp = seg_pos(in.geo, wlo - (BATCH ? grids[gi].seg_first : 0));
+    for (int64_t s = wlo; s < whi; ++s) {  // wave-uniform
+        int nv, nf;
+        uint32_t fl, iw;
+        if constexpr (BATCH) {
+            if (gi + 1 < src.ngrids && s == grids[gi + 1].seg_first) {  // next grid
+                gfl = wave_or(gfl);
+                if (lane == 0 && gfl) atomicOr(src.gflags + gi, gfl);
+                gfl = 0;
+                ++gi;
+                in = load_uniform(&grids[gi].in);
+                sp = SegPos{0, 0, 0};
             }
-            wv += tv;
-            wf += tf;
-            wfl |= fl;
         }
+        CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
+        w.template load_box<0, 2, 0, 2, 0, 2>(in);
+        if constexpr (BATCH) {
+            const int algo = grids[gi].algo;
+            count_position<S>(in, algo, s_case[algo == DISO_MC ? 0 : (BATCH ? 1 : 0)], w, &nv, &nf, &fl, &iw);
+            gfl |= fl;
+        } else {
+            count_position<S>(in, ALGO, s_case[0], w, &nv, &nf, &fl, &iw);
+        }
+        if (++sp.chunk == in.geo.nchunk) {  // next segment of the grid
+            sp.chunk = 0;
+            if (++sp.py == in.geo.g1) {
+                sp.py = 0;
+                ++sp.px;
+            }
+        }
+        info[s * kWave + lane] = (uint16_t)iw;
+        const int tv = wave_total_small(nv), tf = wave_total_small(nf);
+        if (lane == 0) {
+            seg[s].v = tv;
+            seg[s].f = tf;
+        }
+        wv += tv;
+        wf += tf;
+        wfl |= fl;
+    }
+    if constexpr (BATCH) {
+        gfl = wave_or(gfl);
+        if (lane == 0 && gfl) atomicOr(src.gflags + gi, gfl);
     }
     wfl = wave_or(wfl);
     if (lane == 0) {
@@ -577,7 +656,7 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *s
         efl |= s_fl[w];
     }
     if (b == nblocks - 1 && tid == 0) {
-        SegBase *tot = seg + g.nseg;
+        SegBase *tot = seg + nseg;
         tot[0].v = ev + av;
         tot[0].f = ef + af;
         tot[1].v = (int64_t)(efl | afl);
@@ -643,6 +722,19 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(In<S> in, SegBase *s
 // then processes the list 64 items at a time. Every item is computed exactly once (a lane-per-
 // cell loop would make the whole wave execute the union of all lanes' branches), and item k is
 // the segment's k-th output row, so consecutive lanes write consecutive rows (coalesced stores).
+
+// Emission and backward blocks each take a run of `run` (<= 64) consecutive segments (so per-block
+// setup, such as the case tables in LDS, is paid once per run) and deal its segments with work
+// round-robin to their waves; the rest cost one lane-parallel load of the run's bases. The run
+// length is a launch parameter (diso_set_param).
+constexpr int kMaxRun = kWave;
+
+// Bit i: segment lo + i has faces (MC triangles / DMC patches), i.e. seg[s + 1].f != seg[s].f
+// (the entry after the last segment is the totals entry).
+__device__ __forceinline__ uint64_t run_active(const SegBase *__restrict__ seg, int64_t lo, int64_t hi, int lane) {
+    const int64_t s = lo + lane;
+    return __ballot(s < hi && seg[s + 1].f != seg[s].f);
+}
 
 struct OutCfg {
     int flags;
@@ -718,29 +810,22 @@ struct McEmitLds {
     uint8_t vert[kWave * 3];
 };
 
+// One segment that has triangles (hence cells, so rows +x, +y, +x+y exist).
 template <typename S, bool DEF>
-__global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase *__restrict__ seg,
-                                                         S *__restrict__ verts, void *__restrict__ faces,
-                                                         OutCfg out, Norm<S> nm, int64_t nblocks) {
-    __shared__ uint64_t s_case[256];
-    __shared__ McEmitLds s_lds[kWavesPerBlock];
-    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
-    for (int i = tid; i < 256; i += kBlock) s_case[i] = kMcCase[i];
-    __syncthreads();
+__device__ __forceinline__ void emit_mc_segment(const In<S> &in, const SegCtx &sc, int64_t s, SegPos sp,
+                                                S *__restrict__ verts, void *__restrict__ faces, const OutCfg &out,
+                                                const Norm<S> &nm, const uint64_t *s_case, McEmitLds &L,
+                                                int lane) {
+    wave_lds_sync();  // the previous segment's LDS reads are done
     const Geo &g = in.geo;
-    const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
-    if (s >= g.nseg) return;
-    const int64_t tbase = seg[s].f;
-    if (seg[s + 1].f == tbase) return;  // no triangle (hence no owned crossing) in this segment
-    const SegPos sp = seg_pos(g, s);
+    const SegBase *__restrict__ seg = sc.seg;
+    const int64_t tbase = seg[s].f - sc.f0;
     const int px = sp.px, py = sp.py, chunk = sp.chunk;
     const int z0 = chunk * kWave, pz = z0 + lane;
-    const int64_t vbase = seg[s].v;
-    const SegBase *tot = seg + g.nseg;
-    if (tot[0].v > out.vcap || tot[0].f > out.fcap) return;
+    const int64_t vabs = seg[s].v, vbase = vabs - sc.v0;
 
     // This row has cells, so rows +x, +y, +x+y exist; +z words are read within the row only.
-    const uint16_t *info = info_of(seg, g);
+    const uint16_t *info = sc.info;
     const int64_t s10 = seg_index(g, px + 1, py, chunk), s01 = seg_index(g, px, py + 1, chunk),
                   s11 = seg_index(g, px + 1, py + 1, chunk);
     const uint32_t w00 = info[s * kWave + lane];
@@ -757,11 +842,10 @@ __global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase
     int tv, tt, t;
     const int p00 = wave_excl_small(popc3(m00), &tv);
     const int pt = wave_excl_small(ntri, &tt);
-    const int32_t r10 = (int32_t)(seg[s10].v - vbase) + wave_excl_small(popc3(m10), &t);
-    const int32_t r01 = (int32_t)(seg[s01].v - vbase) + wave_excl_small(popc3(m01), &t);
-    const int32_t r11 = (int32_t)(seg[s11].v - vbase) + wave_excl_small(popc3(m11), &t);
+    const int32_t r10 = (int32_t)(seg[s10].v - vabs) + wave_excl_small(popc3(m10), &t);
+    const int32_t r01 = (int32_t)(seg[s01].v - vabs) + wave_excl_small(popc3(m01), &t);
+    const int32_t r11 = (int32_t)(seg[s11].v - vabs) + wave_excl_small(popc3(m11), &t);
 
-    McEmitLds &L = s_lds[wave];
     {
         int r = p00;
 #pragma unroll
@@ -786,6 +870,31 @@ __global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase
     }
 }
 
+template <typename S, bool DEF>
+__global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase *__restrict__ seg,
+                                                         S *__restrict__ verts, void *__restrict__ faces,
+                                                         OutCfg out, Norm<S> nm, int run, int64_t nblocks) {
+    __shared__ uint64_t s_case[256];
+    __shared__ McEmitLds s_lds[kWavesPerBlock];
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
+    for (int i = tid; i < 256; i += kBlock) s_case[i] = kMcCase[i];
+    __syncthreads();
+    const Geo &g = in.geo;
+    const SegCtx sc{seg, info_of(seg, g), 0, 0};
+    const SegBase *tot = seg + g.nseg;
+    if (tot[0].v > out.vcap || tot[0].f > out.fcap) return;
+    const int64_t lo = xcd_block(blockIdx.x, nblocks) * run, hi = lo + run < g.nseg ? lo + run : g.nseg;
+    // segments of the run with triangles (hence owned crossings), dealt round-robin to the waves
+    uint64_t act = run_active(seg, lo, hi, lane);
+    const SegPos lp = seg_pos(g, lo + lane);
+    for (int k = 0; act; ++k, act &= act - 1)
+        if ((k & (kWavesPerBlock - 1)) == wave) {
+            const int i = __builtin_ctzll(act);
+            emit_mc_segment<S, DEF>(in, sc, lo + i, pos_of_lane(lp, i), verts, faces, out, nm, s_case, s_lds[wave],
+                                    lane);
+        }
+}
+
 // DMC emission. Patch-vertex item = (lane, patch): the mean of the MC vertices on the patch's
 // edges, summed from zero in ascending edge order and scaled by 1/num (src/cudualmc.cu:907-955);
 // each MC vertex is evaluated with the same expression as the MC pass, so it has the same bits.
@@ -805,11 +914,32 @@ __device__ constexpr int stencil_eid(int ty, int k) {
     return ty < 3 ? q[ty][k] : q[ty - 3][(4 - k) & 3];
 }
 
+// Edge kinds of a segment's cells, by owner row (dx, dy) and axis: 0: row 00 x, 1: row 00 y,
+// 2: row 00 z, 3: row 10 y, 4: row 10 z, 5: row 01 x, 6: row 01 z, 7: row 11 z (the owner of an
+// edge is the position it starts from; row dxy is (px + dx, py + dy)). The x and y edges of the
+// last cell's top face start at z0 + 64 (the "halo", kinds 8..11 for 0, 1, 3, 5 there).
+// Cell edge e (table numbering) as (kind | top << 3) in 5 bits: top = the edge starts at z + 1.
+__device__ constexpr int edge_kind(int e) {
+    return eax(e) == 0 ? (eoy(e) ? 5 : 0) : eax(e) == 1 ? (eox(e) ? 3 : 1) : (eox(e) + 2 * eoy(e) == 0 ? 2 : eox(e) + 2 * eoy(e) == 1 ? 4 : eox(e) + 2 * eoy(e) == 2 ? 6 : 7);
+}
+__device__ constexpr uint64_t edge_kind_table() {
+    uint64_t t = 0;
+    for (int e = 0; e < 12; ++e) t |= (uint64_t)(edge_kind(e) | (eoz(e) << 3)) << (5 * e);
+    return t;
+}
+constexpr int kEdgeSlots = 320;  // MC vertices kept per segment; beyond, patches recompute theirs
+
+template <typename S>
 struct DmcEmitLds {
     uint8_t code[8][kWave];    // codes of the cells (-i,-j,-k) around the lane, ci = 4i + 2j + k
     int32_t base[8][kWave];    // their first DMC vertex, relative to the segment's first
     uint8_t patch[kWave * 4];  // patch items: lane * 4 + patch
     uint16_t quad[kWave * 3];  // quad items: lane * 8 + axis * 2 + leaving
+    uint8_t ex[8][kWave];      // crossing edges of each kind at the lanes before
+    uint8_t kb[kWave];         // bit k: the lane's edge of kind k crosses
+    int16_t kbase[12];         // first slot of each kind
+    uint16_t eitem[kEdgeSlots];  // edge items: lane << 4 | kind
+    S ev[3][kEdgeSlots];       // their MC vertices
 };
 
 // Edge masks of the patches of every DMC case, 16 bits per patch (patch k in bits 16k..16k+11),
@@ -856,31 +986,32 @@ __device__ __forceinline__ void load_dmc_tables(DmcTables<S> &T, int tid) {
     }
 }
 
+// One segment that has patches (hence cells, so rows +x, +y exist).
 template <typename S, bool DEF>
-__global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBase *__restrict__ seg,
-                                                          S *__restrict__ verts, void *__restrict__ faces,
-                                                          uint8_t *__restrict__ vinfo, OutCfg out, Norm<S> nm,
-                                                          int64_t nblocks) {
-    __shared__ DmcTables<S> T;
-    __shared__ DmcEmitLds s_lds[kWavesPerBlock];
-    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
-    load_dmc_tables(T, tid);
-    __syncthreads();
+__device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &sc, int64_t s, SegPos sp,
+                                                 S *__restrict__ verts, void *__restrict__ faces,
+                                                 uint8_t *__restrict__ vinfo, const OutCfg &out, const Norm<S> &nm,
+                                                 const DmcTables<S> &T, DmcEmitLds<S> &L, int lane) {
+    wave_lds_sync();  // the previous segment's LDS reads are done
     const Geo &g = in.geo;
-    const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
-    if (s >= g.nseg) return;
-    const int64_t pbase = seg[s].f;
-    if (seg[s + 1].f == pbase) return;  // no patch (hence no owned crossing) in this segment
-    const SegPos sp = seg_pos(g, s);
+    const SegBase *__restrict__ seg = sc.seg;
+    const int64_t pabs = seg[s].f, pbase = pabs - sc.f0;
     const int px = sp.px, py = sp.py, chunk = sp.chunk;
     const int z0 = chunk * kWave, pz = z0 + lane;
-    const int64_t qbase = seg[s].v;
-    const uint16_t *info = info_of(seg, g);
+    const int64_t qbase = seg[s].v - sc.v0;
+    const uint16_t *info = sc.info;
     const uint32_t w0 = info[s * kWave + lane];
-    const SegBase *tot = seg + g.nseg;
-    if (tot[0].v > out.fcap || tot[0].f > out.vcap) return;
-
-    DmcEmitLds &L = s_lds[wave];
+    // Crossing masks of the rows the cells' edges start from: 10, 01, 11 (this z) and the halo.
+    const int64_t s10 = seg_index(g, px + 1, py, chunk), s01 = seg_index(g, px, py + 1, chunk),
+                  s11 = seg_index(g, px + 1, py + 1, chunk);
+    const int m00 = info_mask(w0), m10 = info_mask(info[s10 * kWave + lane]),
+              m01 = info_mask(info[s01 * kWave + lane]), m11 = info_mask(info[s11 * kWave + lane]);
+    int h00 = 0, h10 = 0, h01 = 0;
+    if (lane == kWave - 1 && pz + 1 < g.g2) {  // position z0 + 64, in the row's next segment
+        h00 = info_mask(info[s * kWave + lane + 1]);
+        h10 = info_mask(info[s10 * kWave + lane + 1]);
+        h01 = info_mask(info[s01 * kWave + lane + 1]);
+    }
     // Codes and first vertices of the cells (-i, -j, 0) and (-i, -j, -1).
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -891,13 +1022,12 @@ __global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBas
             const int cur = (i || j) ? (row ? info_code(info[r * kWave + lane]) : 0) : info_code(w0);
             const int prev = (row && !(i && j) && pz >= 1) ? info_code(info[r * kWave + lane - 1]) : 0;
             int t;
-            const int32_t b = (int32_t)((row ? seg[r].f : 0) - pbase) + wave_excl_small(DISO_DMC_NPATCH(T.cases[cur]), &t);
+            const int32_t b = (int32_t)((row ? seg[r].f : pabs) - pabs) + wave_excl_small(DISO_DMC_NPATCH(T.cases[cur]), &t);
             L.code[4 * i + 2 * j][lane] = (uint8_t)cur;
             L.base[4 * i + 2 * j][lane] = b;
             L.code[4 * i + 2 * j + 1][lane] = (uint8_t)prev;
             L.base[4 * i + 2 * j + 1][lane] = b - DISO_DMC_NPATCH(T.cases[prev]);
         }
-    const int m00 = info_mask(w0);
     const int np = DISO_DMC_NPATCH(T.cases[info_code(w0)]);
     int tq, tp;
     const int pq = wave_excl_small(popc3(m00), &tq);
@@ -912,21 +1042,76 @@ __global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBas
         for (int a = 0; a < 3; ++a)
             if ((m00 >> a) & 1) L.quad[q++] = (uint16_t)(lane * 8 + a * 2 + leaving);
     }
+    // Edge slots: kinds 0..7 at every lane, then the halo kinds 8..11 (lane 63 only).
+    const int kbits = (m00 & 7) | (((m10 >> 1) & 3) << 3) | ((m01 & 1) << 5) | (((m01 >> 2) & 1) << 6) |
+                      (((m11 >> 2) & 1) << 7);
+    const int hbits = (h00 & 3) | (((h10 >> 1) & 1) << 2) | ((h01 & 1) << 3);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int nslot = 0, myslot[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        const bool on = k < 8 ? ((kbits >> k) & 1) : ((hbits >> (k - 8)) & 1);
+        const uint64_t bal = __ballot(on);
+        const int ex = __popcll(bal & lt);
+        if (k < 8) L.ex[k][lane] = (uint8_t)ex;
+        if (lane == 0) L.kbase[k] = (int16_t)nslot;
+        myslot[k] = on ? nslot + ex : -1;
+        nslot += __popcll(bal);
+    }
+    L.kb[lane] = (uint8_t)kbits;
+    const bool slots = nslot <= kEdgeSlots;  // wave-uniform
+    if (slots) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            if (myslot[k] >= 0) L.eitem[myslot[k]] = (uint16_t)((lane << 4) | k);
+    }
     wave_lds_sync();
 
+    if (slots) {
+        // MC vertex of every crossing edge of the cells, once
+        for (int i = lane; i < nslot; i += kWave) {
+            const int it = L.eitem[i], l = it >> 4, k0 = it & 15;
+            const int k = k0 < 8 ? k0 : (k0 == 8 ? 0 : (k0 == 9 ? 1 : (k0 == 10 ? 3 : 5)));
+            // kinds 3, 4, 7 start at dx = 1; 5, 6, 7 at dy = 1; 2, 4, 6, 7 run along z, 1 and 3 along y
+            const int dx = (0x98 >> k) & 1, dy = (0xE0 >> k) & 1, a = (0xD4 >> k) & 1 ? 2 : (0x0A >> k) & 1;
+            const V3<S> p = edge_vertex<S, DEF>(in, px + dx, py + dy, z0 + (k0 < 8 ? l : kWave), a);
+            L.ev[0][i] = p.x;
+            L.ev[1][i] = p.y;
+            L.ev[2][i] = p.z;
+        }
+        wave_lds_sync();
+    }
+
+    constexpr uint64_t kKinds = edge_kind_table();
     for (int k = lane; k < tp; k += kWave) {
         const int it = L.patch[k], src = it >> 2, pk = it & 3;
         unsigned em = (unsigned)(T.pedges[L.code[0][src]] >> (16 * pk)) & 0xfffu;
         const int num = __popc(em);
-        const int z = z0 + src;
         S sx = S(0), sy = S(0), sz = S(0);
-        while (em) {  // ascending edge order
-            const int e = __ffs(em) - 1;
-            em &= em - 1;
-            const V3<S> p = edge_vertex<S, DEF>(in, px + eox(e), py + eoy(e), z + eoz(e), eax(e));
-            sx += p.x;
-            sy += p.y;
-            sz += p.z;
+        if (slots) {
+            const int bits = L.kb[src];
+            while (em) {  // ascending edge order
+                const int e = __ffs(em) - 1;
+                em &= em - 1;
+                const int v = (int)(kKinds >> (5 * e)) & 31, kd = v & 7, top = v >> 3;
+                int slot = (top && src == kWave - 1) ? L.kbase[8 + ((kd + 1) >> 1)]
+                                                     : L.kbase[kd] + L.ex[kd][src] + (top & (bits >> kd));
+                slot = slot < nslot ? slot : nslot - 1;  // only a NaN sample can make a patch edge not cross
+
+                sx += L.ev[0][slot];
+                sy += L.ev[1][slot];
+                sz += L.ev[2][slot];
+            }
+        } else {
+            const int z = z0 + src;
+            while (em) {  // ascending edge order
+                const int e = __ffs(em) - 1;
+                em &= em - 1;
+                const V3<S> p = edge_vertex<S, DEF>(in, px + eox(e), py + eoy(e), z + eoz(e), eax(e));
+                sx += p.x;
+                sy += p.y;
+                sz += p.z;
+            }
         }
         const S r = T.rcp[num];
         store_vertex(verts, pbase + k, out.vcap, epilogue(V3<S>{sx * r, sy * r, sz * r}, out.flags, nm));
@@ -944,6 +1129,32 @@ __global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBas
             store_index(faces, 4 * q + j, pbase + L.base[ci][src] + (p == 7 ? -1 : p), out.flags);
         }
     }
+}
+
+template <typename S, bool DEF>
+__global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBase *__restrict__ seg,
+                                                          S *__restrict__ verts, void *__restrict__ faces,
+                                                          uint8_t *__restrict__ vinfo, OutCfg out, Norm<S> nm,
+                                                          int run, int64_t nblocks) {
+    __shared__ DmcTables<S> T;
+    __shared__ DmcEmitLds<S> s_lds[kWavesPerBlock];
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
+    load_dmc_tables(T, tid);
+    __syncthreads();
+    const Geo &g = in.geo;
+    const SegCtx sc{seg, info_of(seg, g), 0, 0};
+    const SegBase *tot = seg + g.nseg;
+    if (tot[0].v > out.fcap || tot[0].f > out.vcap) return;
+    const int64_t lo = xcd_block(blockIdx.x, nblocks) * run, hi = lo + run < g.nseg ? lo + run : g.nseg;
+    // segments of the run with patches (hence owned crossings), dealt round-robin to the waves
+    uint64_t act = run_active(seg, lo, hi, lane);
+    const SegPos lp = seg_pos(g, lo + lane);
+    for (int k = 0; act; ++k, act &= act - 1)
+        if ((k & (kWavesPerBlock - 1)) == wave) {
+            const int i = __builtin_ctzll(act);
+            emit_dmc_segment<S, DEF>(in, sc, lo + i, pos_of_lane(lp, i), verts, faces, vinfo, out, nm, T, s_lds[wave],
+                                     lane);
+        }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1077,30 +1288,30 @@ __device__ __forceinline__ bool seg_owns_crossing(const SegBase *__restrict__ se
     return t >= 0 && seg[t + 1].v != seg[t].v;
 }
 
+// The edges incident to a segment's vertices are owned by the segment itself, by the segments of
+// rows -x and -y, and (the z edge into lane 0) by the previous chunk: whether any of them owns a
+// crossing edge.
+__device__ __forceinline__ bool seg_touched(const SegBase *__restrict__ seg, const Geo &g, int64_t s, SegPos sp) {
+    return seg_owns_crossing(seg, s) || (sp.px > 0 && seg_owns_crossing(seg, seg_index(g, sp.px - 1, sp.py, sp.chunk))) ||
+           (sp.py > 0 && seg_owns_crossing(seg, seg_index(g, sp.px, sp.py - 1, sp.chunk))) ||
+           (sp.chunk > 0 && seg_owns_crossing(seg, s - 1));
+}
+
+// Gather of one touched segment.
 template <typename S, bool DEF, bool RECOMP>
-__global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase *__restrict__ seg, AdjIn<S> adj,
-                                                        S *__restrict__ adj_sdf, S *__restrict__ adj_deform,
-                                                        int64_t nblocks) {
-    __shared__ BwdLds<S> s_lds[kWavesPerBlock];
-    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
+__device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, int64_t s, SegPos sp,
+                                            const AdjIn<S> &adj, S *__restrict__ adj_sdf,
+                                            S *__restrict__ adj_deform, BwdLds<S> &L, int lane) {
+    wave_lds_sync();  // the previous segment's LDS reads are done
     const Geo &g = in.geo;
-    const int64_t s = xcd_block(blockIdx.x, nblocks) * kWavesPerBlock + wave;
-    if (s >= g.nseg) return;
-    const SegPos sp = seg_pos(g, s);
+    const SegBase *__restrict__ seg = sc.seg;
     const int px = sp.px, py = sp.py, chunk = sp.chunk;
     const int z0 = chunk * kWave, pz = z0 + lane;
     const S iso = in.iso;
-    // The edges incident to this segment's vertices are owned by the segment itself, by the
-    // segments of rows -x and -y, and (the z edge into lane 0) by the previous chunk.
     const int64_t sm0 = px > 0 ? seg_index(g, px - 1, py, chunk) : -1;
     const int64_t s0m = py > 0 ? seg_index(g, px, py - 1, chunk) : -1;
     Grad<S> acc = {S(0), S(0), S(0), S(0)};
-    if (!seg_owns_crossing(seg, s) && !seg_owns_crossing(seg, sm0) && !seg_owns_crossing(seg, s0m) &&
-        !(chunk > 0 && seg_owns_crossing(seg, s - 1))) {
-        if (!(adj.flags & DISO_FLAG_ACCUMULATE)) store_grad(in, adj_sdf, adj_deform, px, py, pz, acc, adj.flags);
-        return;
-    }
-    const int64_t vbase = seg[s].v;
+    const int64_t vabs = seg[s].v, vbase = vabs - sc.v0;
     // Crossing masks of the edges owned by the vertex and by its -x, -y, -z neighbours: from the
     // classification words, or (RECOMP) from the samples of the vertex's neighbourhood.
     int m00, mm0, m0m, mzm;
@@ -1119,7 +1330,7 @@ __global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase 
         m0m = is_anchor(in, px, py - 1, pz) ? own_mask(w(0, -1, 0), w(1, -1, 0), c, w(0, -1, 1), iso) : 0;
         mzm = is_anchor(in, px, py, pz - 1) ? crosses(w(0, 0, -1), c, iso) << 2 : 0;
     } else {
-        const uint16_t *info = info_of(seg, g);
+        const uint16_t *info = sc.info;
         m00 = info_mask(info[s * kWave + lane]);
         mm0 = sm0 >= 0 ? info_mask(info[sm0 * kWave + lane]) : 0;
         m0m = s0m >= 0 ? info_mask(info[s0m * kWave + lane]) : 0;
@@ -1130,11 +1341,10 @@ __global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase 
     const int mask = in_data(in, px, py, pz) ? (m00 | (inc_m << 3)) : 0;
     int t, total;
     const int32_t r00 = wave_excl_small(popc3(m00), &t);
-    const int32_t rm0 = (int32_t)((sm0 >= 0 ? seg[sm0].v : 0) - vbase) + wave_excl_small(popc3(mm0), &t);
-    const int32_t r0m = (int32_t)((s0m >= 0 ? seg[s0m].v : 0) - vbase) + wave_excl_small(popc3(m0m), &t);
+    const int32_t rm0 = (int32_t)((sm0 >= 0 ? seg[sm0].v : vabs) - vabs) + wave_excl_small(popc3(mm0), &t);
+    const int32_t r0m = (int32_t)((s0m >= 0 ? seg[s0m].v : vabs) - vabs) + wave_excl_small(popc3(m0m), &t);
     const int first = wave_excl_small(__popc(mask), &total);
 
-    BwdLds<S> &L = s_lds[wave];
     if (mask) {
         L.c[lane] = RECOMP ? c : sample(in, px, py, pz);
         const V3<S> f = deform_at<S, DEF>(in, px, py, pz);
@@ -1186,17 +1396,39 @@ __global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase 
     store_grad(in, adj_sdf, adj_deform, px, py, pz, acc, adj.flags);
 }
 
+template <typename S, bool DEF, bool RECOMP>
+__global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase *__restrict__ seg, AdjIn<S> adj,
+                                                        S *__restrict__ adj_sdf, S *__restrict__ adj_deform,
+                                                        int run, int64_t nblocks) {
+    __shared__ BwdLds<S> s_lds[kWavesPerBlock];
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
+    const Geo &g = in.geo;
+    const SegCtx sc{seg, info_of(seg, g), 0, 0};
+    const int64_t lo = xcd_block(blockIdx.x, nblocks) * run, hi = lo + run < g.nseg ? lo + run : g.nseg;
+    const int64_t t = lo + lane;
+    const SegPos lp = seg_pos(g, t);
+    const bool touched = t < hi && seg_touched(seg, g, t, lp);
+    uint64_t act = __ballot(touched);
+    // untouched segments get zero gradients (unless accumulating), dealt round-robin as well
+    uint64_t idle = (adj.flags & DISO_FLAG_ACCUMULATE) ? 0ull : __ballot(t < hi && !touched);
+    for (int k = 0; idle; ++k, idle &= idle - 1)
+        if ((k & (kWavesPerBlock - 1)) == wave) {
+            const SegPos sp = pos_of_lane(lp, __builtin_ctzll(idle));
+            store_grad(in, adj_sdf, adj_deform, sp.px, sp.py, sp.chunk * kWave + lane, Grad<S>{S(0), S(0), S(0), S(0)},
+                       adj.flags);
+        }
+    for (int k = 0; act; ++k, act &= act - 1)
+        if ((k & (kWavesPerBlock - 1)) == wave)
+            bwd_segment<S, DEF, RECOMP>(in, sc, lo + __builtin_ctzll(act), pos_of_lane(lp, __builtin_ctzll(act)), adj,
+                                        adj_sdf, adj_deform, s_lds[wave], lane);
+}
+
 // Per-edge upstream gradient of a DMC forward: ge[q] = sum over the quad's corners v of
 // (dL/dv) / (edges in v's patch), corners in the quad's order.
 template <typename S>
-__global__ __launch_bounds__(kBlock) void dmc_edge_grad_kernel(const void *__restrict__ quads, int64_t nq,
-                                                               const uint8_t *__restrict__ vinfo, AdjIn<S> adj,
-                                                               S *__restrict__ ge) {
-    __shared__ S s_rcp[8];
-    if (threadIdx.x < 8) s_rcp[threadIdx.x] = threadIdx.x ? S(1) / S(threadIdx.x) : S(0);
-    __syncthreads();
-    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (q >= nq) return;
+__device__ __forceinline__ void edge_grad_one(const void *__restrict__ quads, int64_t q,
+                                              const uint8_t *__restrict__ vinfo, const AdjIn<S> &adj, const S *s_rcp,
+                                              S *__restrict__ ge) {
     const bool compat = (adj.flags & DISO_FLAG_REFERENCE_COMPAT) != 0;
     V3<S> sum = {S(0), S(0), S(0)};
 #pragma unroll
@@ -1213,6 +1445,189 @@ __global__ __launch_bounds__(kBlock) void dmc_edge_grad_kernel(const void *__res
     ge[3 * q + 0] = sum.x;
     ge[3 * q + 1] = sum.y;
     ge[3 * q + 2] = sum.z;
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void dmc_edge_grad_kernel(const void *__restrict__ quads, int64_t nq,
+                                                               const uint8_t *__restrict__ vinfo, AdjIn<S> adj,
+                                                               S *__restrict__ ge) {
+    __shared__ S s_rcp[8];
+    if (threadIdx.x < 8) s_rcp[threadIdx.x] = threadIdx.x ? S(1) / S(threadIdx.x) : S(0);
+    __syncthreads();
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q < nq) edge_grad_one(quads, q, vinfo, adj, s_rcp, ge);
+}
+
+// ------------------------------------------------------------------------------------------
+// Batched extraction: many grids (uneven dims, MC and DMC, with or without deform) per launch.
+//
+// The count pass runs once over the concatenation of all grids' segments (a grid's segments are
+// consecutive, grids in batch order), so one single-pass scan yields every grid's bases; a grid's
+// own output rows are its bases minus those of its first segment. Emission and backward launch
+// one block per run of segments of one grid: a launch covers the runs of every grid of one
+// kind (algorithm x deform x recompute), listed in `gl` with the prefix of their run counts in
+// `rf`; a block finds its grid by binary search over `rf`.
+
+template <typename S>
+struct BatchGrid {
+    In<S> in;
+    Norm<S> nm;
+    int algo, flags;
+    int64_t seg_first;  // first segment in the batch's segment space
+    int64_t nv, nf;     // vertex and face rows (from the count pass)
+    S *verts;
+    void *faces;
+    uint8_t *vinfo;
+    int64_t vcap, fcap;
+    const S *adj;  // upstream vertex gradients [nv, 3]
+    S *adj_sdf, *adj_deform;
+    int64_t q_first;  // DMC: first row of its per-edge gradients in the batch scratch
+};
+
+// j such that rf[j] <= b < rf[j + 1] (rf ascending, rf[0] = 0, rf[m] = total).
+__device__ __forceinline__ int find_group(const int64_t *__restrict__ rf, int m, int64_t b) {
+    int lo = 0, hi = m;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (rf[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <typename S>
+__device__ __forceinline__ SegCtx batch_ctx(const BatchGrid<S> &G, const SegBase *seg, const uint16_t *info) {
+    return SegCtx{seg + G.seg_first, info + G.seg_first * kWave, seg[G.seg_first].v, seg[G.seg_first].f};
+}
+
+template <typename S>
+__global__ void batch_totals_kernel(const BatchGrid<S> *__restrict__ grids, int n, int64_t nseg,
+                                    const SegBase *__restrict__ seg, uint32_t *__restrict__ gflags,
+                                    diso_totals_t *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SegBase a = seg[grids[i].seg_first], e = seg[i + 1 < n ? grids[i + 1].seg_first : nseg];
+    const int64_t cross = e.v - a.v, faces = e.f - a.f;
+    const bool dmc = grids[i].algo == DISO_DMC;
+    out[i].n_verts = dmc ? faces : cross;
+    out[i].n_faces = dmc ? cross : faces;
+    out[i].flags = (int64_t)(gflags[i] | ((uint32_t)seg[nseg + 1].v & DISO_TOTAL_SCAN_TIMEOUT));
+    out[i].n_crossing = cross;
+    gflags[i] = 0u;  // ready for the next batch
+}
+
+template <typename S, bool DEF>
+__global__ __launch_bounds__(kBlock) void emit_mc_batch_kernel(const BatchGrid<S> *__restrict__ grids,
+                                                               const int32_t *__restrict__ gl,
+                                                               const int64_t *__restrict__ rf, int m,
+                                                               const SegBase *__restrict__ seg,
+                                                               const uint16_t *__restrict__ info, int run, int64_t nblocks) {
+    __shared__ uint64_t s_case[256];
+    __shared__ McEmitLds s_lds[kWavesPerBlock];
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
+    for (int i = tid; i < 256; i += kBlock) s_case[i] = kMcCase[i];
+    __syncthreads();
+    const int64_t b = xcd_block(blockIdx.x, nblocks);
+    const int j = find_group(rf, m, b);
+    const BatchGrid<S> &G = grids[gl[j]];
+    if (G.nv > G.vcap || G.nf > G.fcap) return;
+    const In<S> in = G.in;
+    const SegCtx sc = batch_ctx(G, seg, info);
+    const OutCfg out{G.flags, G.vcap, G.fcap};
+    const int64_t lo = (b - rf[j]) * run, hi = lo + run < in.geo.nseg ? lo + run : in.geo.nseg;
+    uint64_t act = run_active(sc.seg, lo, hi, lane);
+    const SegPos lp = seg_pos(in.geo, lo + lane);
+    for (int k = 0; act; ++k, act &= act - 1)
+        if ((k & (kWavesPerBlock - 1)) == wave) {
+            const int i = __builtin_ctzll(act);
+            emit_mc_segment<S, DEF>(in, sc, lo + i, pos_of_lane(lp, i), G.verts, G.faces, out, G.nm, s_case,
+                                    s_lds[wave], lane);
+        }
+}
+
+template <typename S, bool DEF>
+__global__ __launch_bounds__(kBlock) void emit_dmc_batch_kernel(const BatchGrid<S> *__restrict__ grids,
+                                                                const int32_t *__restrict__ gl,
+                                                                const int64_t *__restrict__ rf, int m,
+                                                                const SegBase *__restrict__ seg,
+                                                                const uint16_t *__restrict__ info, int run, int64_t nblocks) {
+    __shared__ DmcTables<S> T;
+    __shared__ DmcEmitLds<S> s_lds[kWavesPerBlock];
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
+    load_dmc_tables(T, tid);
+    __syncthreads();
+    const int64_t b = xcd_block(blockIdx.x, nblocks);
+    const int j = find_group(rf, m, b);
+    const BatchGrid<S> &G = grids[gl[j]];
+    if (G.nv > G.vcap || G.nf > G.fcap) return;
+    const In<S> in = G.in;
+    const SegCtx sc = batch_ctx(G, seg, info);
+    const OutCfg out{G.flags, G.vcap, G.fcap};
+    const int64_t lo = (b - rf[j]) * run, hi = lo + run < in.geo.nseg ? lo + run : in.geo.nseg;
+    uint64_t act = run_active(sc.seg, lo, hi, lane);
+    const SegPos lp = seg_pos(in.geo, lo + lane);
+    for (int k = 0; act; ++k, act &= act - 1)
+        if ((k & (kWavesPerBlock - 1)) == wave) {
+            const int i = __builtin_ctzll(act);
+            emit_dmc_segment<S, DEF>(in, sc, lo + i, pos_of_lane(lp, i), (S *)G.verts, G.faces, G.vinfo, out, G.nm,
+                                     T, s_lds[wave], lane);
+        }
+}
+
+// Per-edge upstream gradients of every DMC grid of the batch: blocks of kBlock quads of one grid.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void dmc_edge_grad_batch_kernel(const BatchGrid<S> *__restrict__ grids,
+                                                                     const int32_t *__restrict__ gl,
+                                                                     const int64_t *__restrict__ rf, int m,
+                                                                     S *__restrict__ ge) {
+    __shared__ S s_rcp[8];
+    if (threadIdx.x < 8) s_rcp[threadIdx.x] = threadIdx.x ? S(1) / S(threadIdx.x) : S(0);
+    __syncthreads();
+    const int64_t b = blockIdx.x;
+    const int j = find_group(rf, m, b);
+    const BatchGrid<S> &G = grids[gl[j]];
+    const int64_t q = (b - rf[j]) * kBlock + threadIdx.x;
+    if (q >= G.nf) return;
+    const AdjIn<S> a{G.adj, G.nv, G.flags, G.nm};
+    edge_grad_one(G.faces, q, G.vinfo, a, s_rcp, ge + 3 * G.q_first);
+}
+
+template <typename S, bool DEF, bool RECOMP>
+__global__ __launch_bounds__(kBlock) void bwd_batch_kernel(const BatchGrid<S> *__restrict__ grids,
+                                                           const int32_t *__restrict__ gl,
+                                                           const int64_t *__restrict__ rf, int m,
+                                                           const SegBase *__restrict__ seg,
+                                                           const uint16_t *__restrict__ info,
+                                                           const S *__restrict__ ge, int run, int64_t nblocks) {
+    __shared__ BwdLds<S> s_lds[kWavesPerBlock];
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
+    const int64_t b = xcd_block(blockIdx.x, nblocks);
+    const int j = find_group(rf, m, b);
+    const BatchGrid<S> &G = grids[gl[j]];
+    const In<S> in = G.in;
+    const Geo &g = in.geo;
+    const SegCtx sc = batch_ctx(G, seg, info);
+    // DMC grids gather the per-edge gradients (already divided by the normalisation)
+    const AdjIn<S> adj = G.algo == DISO_DMC ? AdjIn<S>{ge + 3 * G.q_first, G.nf, G.flags & ~DISO_FLAG_NORMALIZE, G.nm}
+                                            : AdjIn<S>{G.adj, G.nv, G.flags, G.nm};
+    const int64_t lo = (b - rf[j]) * run, hi = lo + run < g.nseg ? lo + run : g.nseg;
+    const int64_t t = lo + lane;
+    const SegPos lp = seg_pos(g, t);
+    const bool touched = t < hi && seg_touched(sc.seg, g, t, lp);
+    uint64_t act = __ballot(touched);
+    uint64_t idle = (adj.flags & DISO_FLAG_ACCUMULATE) ? 0ull : __ballot(t < hi && !touched);
+    for (int k = 0; idle; ++k, idle &= idle - 1)
+        if ((k & (kWavesPerBlock - 1)) == wave) {
+            const SegPos sp = pos_of_lane(lp, __builtin_ctzll(idle));
+            store_grad(in, G.adj_sdf, G.adj_deform, sp.px, sp.py, sp.chunk * kWave + lane,
+                       Grad<S>{S(0), S(0), S(0), S(0)}, adj.flags);
+        }
+    for (int k = 0; act; ++k, act &= act - 1)
+        if ((k & (kWavesPerBlock - 1)) == wave)
+            bwd_segment<S, DEF, RECOMP>(in, sc, lo + __builtin_ctzll(act), pos_of_lane(lp, __builtin_ctzll(act)), adj,
+                                        G.adj_sdf, G.adj_deform, s_lds[wave], lane);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1378,6 +1793,13 @@ int check_hip(hipError_t e, const char *what) {
     return fail(DISO_E_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+// Launch parameters (diso_set_param): emission / backward run length, count blocks per CU.
+struct Params {
+    int run = 32;
+    int count_per_cu = kCountBlocksPerCU;
+};
+Params g_params;
+
 bool make_geo(int64_t X, int64_t Y, int64_t Z, int flags, Geo *g) {
     if (X < 1 || Y < 1 || Z < 1) return false;
     const int64_t lim = 1 << 20;
@@ -1415,12 +1837,12 @@ int cu_count() {
     return cus[dev];
 }
 
-int64_t count_blocks(const Geo &g) {
-    int64_t nb = (int64_t)cu_count() * kCountBlocksPerCU;
-    const int64_t min_nb = (g.nseg + (1ll << 20) - 1) >> 20;  // <= 2^20 segments per block keeps the
-    if (nb < min_nb) nb = min_nb;                            // aggregate fields from overflowing
+int64_t count_blocks(int64_t nseg) {
+    int64_t nb = (int64_t)cu_count() * g_params.count_per_cu;
+    const int64_t min_nb = (nseg + (1ll << 20) - 1) >> 20;  // <= 2^20 segments per block keeps the
+    if (nb < min_nb) nb = min_nb;                          // aggregate fields from overflowing
     if (nb > kMaxCountBlocks) nb = kMaxCountBlocks;
-    if (nb > g.nseg) nb = g.nseg;
+    if (nb > nseg) nb = nseg;
     return nb < 1 ? 1 : nb;
 }
 
@@ -1438,19 +1860,23 @@ In<S> make_in(const void *sdf, const void *deform, double iso, const Geo &g) {
 template <typename S>
 int launch_count(int algo, const void *sdf, double iso, const Geo &g, void *seg, void *ctl,
                  int64_t ctl_bytes, hipStream_t st) {
-    In<S> in = make_in<S>(sdf, nullptr, iso, g);
-    const int64_t nb = count_blocks(g);
+    CountSrc<S> src{};
+    src.in = make_in<S>(sdf, nullptr, iso, g);
+    src.nseg = g.nseg;
+    const int64_t nb = count_blocks(g.nseg);
     (void)ctl_bytes;
+    SegBase *sb = (SegBase *)seg;
+    uint16_t *info = reinterpret_cast<uint16_t *>(sb + g.nseg + 2);
     if (algo == DISO_MC)
-        hipLaunchKernelGGL((count_kernel<S, DISO_MC>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, in,
-                           (SegBase *)seg, (uint8_t *)ctl, nb);
+        hipLaunchKernelGGL((count_kernel<S, DISO_MC, false>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, src,
+                           (const BatchGrid<S> *)nullptr, sb, info, (uint8_t *)ctl, nb);
     else
-        hipLaunchKernelGGL((count_kernel<S, DISO_DMC>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, in,
-                           (SegBase *)seg, (uint8_t *)ctl, nb);
+        hipLaunchKernelGGL((count_kernel<S, DISO_DMC, false>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, src,
+                           (const BatchGrid<S> *)nullptr, sb, info, (uint8_t *)ctl, nb);
     return check_hip(hipGetLastError(), "count kernel launch");
 }
 
-int64_t row_blocks(const Geo &g) { return (g.nseg + kWavesPerBlock - 1) / kWavesPerBlock; }
+int64_t row_blocks(const Geo &g) { return (g.nseg + g_params.run - 1) / g_params.run; }
 
 template <typename S>
 Norm<S> norm_of(const Geo &g) {
@@ -1472,20 +1898,21 @@ int launch_emit(int algo, const void *sdf, const void *deform, double iso, int f
     const SegBase *sb = (const SegBase *)seg;
     const OutCfg out{flags, vcap, fcap};
     const dim3 grid((unsigned)nb), block(kBlock);
+    const int run = g_params.run;
     if (algo == DISO_MC) {
         if (deform)
             hipLaunchKernelGGL((emit_mc_kernel<S, true>), grid, block, 0, st, in, sb, (S *)verts, faces, out, nm,
-                               nb);
+                               run, nb);
         else
             hipLaunchKernelGGL((emit_mc_kernel<S, false>), grid, block, 0, st, in, sb, (S *)verts, faces, out, nm,
-                               nb);
+                               run, nb);
     } else {
         if (deform)
             hipLaunchKernelGGL((emit_dmc_kernel<S, true>), grid, block, 0, st, in, sb, (S *)verts, faces,
-                               (uint8_t *)vinfo, out, nm, nb);
+                               (uint8_t *)vinfo, out, nm, run, nb);
         else
             hipLaunchKernelGGL((emit_dmc_kernel<S, false>), grid, block, 0, st, in, sb, (S *)verts, faces,
-                               (uint8_t *)vinfo, out, nm, nb);
+                               (uint8_t *)vinfo, out, nm, run, nb);
     }
     return check_hip(hipGetLastError(), "emit kernel launch");
 }
@@ -1519,13 +1946,13 @@ int launch_backward(int algo, const void *sdf, const void *deform, double iso, i
     const dim3 grid((unsigned)nb), block(kBlock);
     const bool rec = (flags & DISO_FLAG_RECOMPUTE) != 0;
     if (deform && rec)
-        hipLaunchKernelGGL((bwd_mc_kernel<S, true, true>), grid, block, 0, st, in, sb, a, gs, gd, nb);
+        hipLaunchKernelGGL((bwd_mc_kernel<S, true, true>), grid, block, 0, st, in, sb, a, gs, gd, g_params.run, nb);
     else if (deform)
-        hipLaunchKernelGGL((bwd_mc_kernel<S, true, false>), grid, block, 0, st, in, sb, a, gs, gd, nb);
+        hipLaunchKernelGGL((bwd_mc_kernel<S, true, false>), grid, block, 0, st, in, sb, a, gs, gd, g_params.run, nb);
     else if (rec)
-        hipLaunchKernelGGL((bwd_mc_kernel<S, false, true>), grid, block, 0, st, in, sb, a, gs, gd, nb);
+        hipLaunchKernelGGL((bwd_mc_kernel<S, false, true>), grid, block, 0, st, in, sb, a, gs, gd, g_params.run, nb);
     else
-        hipLaunchKernelGGL((bwd_mc_kernel<S, false, false>), grid, block, 0, st, in, sb, a, gs, gd, nb);
+        hipLaunchKernelGGL((bwd_mc_kernel<S, false, false>), grid, block, 0, st, in, sb, a, gs, gd, g_params.run, nb);
     return check_hip(hipGetLastError(), "backward kernel launch");
 }
 
@@ -1578,6 +2005,261 @@ int read_totals(const Geo &g, const void *seg, diso_totals_t *t, hipStream_t st)
     t->n_faces = dmc ? tot[0].v : tot[0].f;
     t->flags = tot[1].v;
     t->n_crossing = tot[0].v;
+    return DISO_OK;
+}
+
+// ---- batches
+
+int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
+
+constexpr int kBatchGroups = 9;  // emit: MC/DMC x deform; backward: deform x recompute; DMC edges
+
+// Where everything of a batch lives in its workspace: the device descriptors, the launch-group
+// tables, the per-grid totals, then the segment bases and classification words of all grids.
+struct BatchLayout {
+    int n = 0;
+    int64_t nseg = 0;
+    std::vector<Geo> geo;
+    std::vector<int64_t> seg_first;
+    int64_t off_desc = 0, off_groups = 0, off_totals = 0, off_seg = 0, off_info = 0, total = 0;
+};
+
+int64_t group_bytes(int n) { return align256((int64_t)n * 4) + align256((int64_t)(n + 1) * 8); }
+
+bool batch_layout(const diso_grid_t *grids, int n, BatchLayout *L) {
+    if (!grids || n < 1) return false;
+    L->n = n;
+    L->geo.resize(n);
+    L->seg_first.resize(n);
+    int64_t nseg = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!make_geo(grids[i].X, grids[i].Y, grids[i].Z, grids[i].flags, &L->geo[i])) return false;
+        L->seg_first[i] = nseg;
+        nseg += L->geo[i].nseg;
+    }
+    if (nseg >= (1ll << 32)) return false;
+    L->nseg = nseg;
+    L->off_desc = 0;
+    L->off_groups = align256((int64_t)n * (int64_t)sizeof(BatchGrid<double>));
+    L->off_totals = L->off_groups + kBatchGroups * group_bytes(n);
+    L->off_seg = L->off_totals + align256((int64_t)n * (int64_t)sizeof(diso_totals_t));
+    L->off_info = L->off_seg + (nseg + 2) * (int64_t)sizeof(SegBase);
+    L->total = L->off_info + nseg * kWave * 2;
+    return true;
+}
+
+int64_t batch_ctl_bytes(int n) { return align256(ctl_bytes_for(Geo{})) + align256((int64_t)n * 4); }
+
+int check_batch(int dtype, const diso_grid_t *grids, int n, BatchLayout *L) {
+    if (dtype != DISO_F32 && dtype != DISO_F64) return fail(DISO_E_INVALID, "dtype must be DISO_F32 or DISO_F64");
+    if (!grids || n < 1) return fail(DISO_E_INVALID, "empty batch");
+    for (int i = 0; i < n; ++i) {
+        if (grids[i].algo != DISO_MC && grids[i].algo != DISO_DMC)
+            return fail(DISO_E_INVALID, "grid %d: algo must be DISO_MC or DISO_DMC", i);
+        if (!grids[i].sdf) return fail(DISO_E_INVALID, "grid %d: null sdf", i);
+    }
+    if (!batch_layout(grids, n, L)) return fail(DISO_E_INVALID, "unsupported grid dims in the batch");
+    return DISO_OK;
+}
+
+// Device descriptors of a batch, as far as the host knows them at this point of the pipeline.
+template <typename S>
+std::vector<BatchGrid<S>> batch_desc(const diso_grid_t *grids, const BatchLayout &L) {
+    std::vector<BatchGrid<S>> d(L.n);
+    int64_t q = 0;
+    for (int i = 0; i < L.n; ++i) {
+        const diso_grid_t &G = grids[i];
+        BatchGrid<S> &D = d[i];
+        memset(&D, 0, sizeof(D));
+        D.in = make_in<S>(G.sdf, G.deform, G.iso, L.geo[i]);
+        D.nm = norm_of<S>(L.geo[i]);
+        D.algo = G.algo;
+        D.flags = G.flags;
+        D.seg_first = L.seg_first[i];
+        D.nv = G.totals.n_verts;
+        D.nf = G.totals.n_faces;
+        D.verts = (S *)G.verts;
+        D.faces = G.faces;
+        D.vinfo = (uint8_t *)G.vinfo;
+        D.vcap = G.vert_capacity;
+        D.fcap = G.face_capacity;
+        D.adj = (const S *)G.adj_verts;
+        D.adj_sdf = (S *)G.adj_sdf;
+        D.adj_deform = (S *)G.adj_deform;
+        D.q_first = q;
+        if (G.algo == DISO_DMC) q += G.totals.n_faces;
+    }
+    return d;
+}
+
+// One launch group: the grids it covers and the prefix of their block counts.
+struct Group {
+    std::vector<int32_t> gl;
+    std::vector<int64_t> rf{0};
+    void add(int i, int64_t blocks) {
+        if (blocks <= 0) return;
+        gl.push_back(i);
+        rf.push_back(rf.back() + blocks);
+    }
+    int m() const { return (int)gl.size(); }
+    int64_t blocks() const { return rf.back(); }
+};
+
+// Host staging of descriptors + group tables, copied into the workspace in one transfer.
+template <typename S>
+int upload_batch(void *ws, const BatchLayout &L, const std::vector<BatchGrid<S>> &d, const Group *groups, int ng,
+                 hipStream_t st) {
+    std::vector<uint8_t> h(L.off_totals, 0);
+    memcpy(h.data() + L.off_desc, d.data(), d.size() * sizeof(BatchGrid<S>));
+    for (int k = 0; k < ng; ++k) {
+        uint8_t *base = h.data() + L.off_groups + k * group_bytes(L.n);
+        memcpy(base, groups[k].gl.data(), groups[k].gl.size() * 4);
+        memcpy(base + align256((int64_t)L.n * 4), groups[k].rf.data(), groups[k].rf.size() * 8);
+    }
+    int rc = check_hip(hipMemcpyAsync(ws, h.data(), h.size(), hipMemcpyHostToDevice, st), "batch descriptor copy");
+    if (rc) return rc;
+    // the staging buffer is pageable and goes out of scope: wait for the copy
+    return check_hip(hipStreamSynchronize(st), "batch descriptor copy");
+}
+
+template <typename S>
+struct BatchPtrs {
+    BatchGrid<S> *desc;
+    SegBase *seg;
+    uint16_t *info;
+    diso_totals_t *totals;
+    const int32_t *gl(const BatchLayout &L, int k, uint8_t *ws) const {
+        return (const int32_t *)(ws + L.off_groups + k * group_bytes(L.n));
+    }
+    const int64_t *rf(const BatchLayout &L, int k, uint8_t *ws) const {
+        return (const int64_t *)(ws + L.off_groups + k * group_bytes(L.n) + align256((int64_t)L.n * 4));
+    }
+};
+
+template <typename S>
+BatchPtrs<S> batch_ptrs(void *ws, const BatchLayout &L) {
+    uint8_t *b = (uint8_t *)ws;
+    return BatchPtrs<S>{(BatchGrid<S> *)(b + L.off_desc), (SegBase *)(b + L.off_seg), (uint16_t *)(b + L.off_info),
+                        (diso_totals_t *)(b + L.off_totals)};
+}
+
+template <typename S>
+int batch_count(diso_grid_t *grids, const BatchLayout &L, void *ws, void *ctl, hipStream_t st) {
+    const std::vector<BatchGrid<S>> d = batch_desc<S>(grids, L);
+    int rc = upload_batch<S>(ws, L, d, nullptr, 0, st);
+    if (rc) return rc;
+    const BatchPtrs<S> P = batch_ptrs<S>(ws, L);
+    CountSrc<S> src{};
+    src.gflags = (uint32_t *)((uint8_t *)ctl + align256(ctl_bytes_for(Geo{})));
+    src.ngrids = L.n;
+    src.nseg = L.nseg;
+    const int64_t nb = count_blocks(L.nseg);
+    hipLaunchKernelGGL((count_kernel<S, DISO_MC, true>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, src,
+                       (const BatchGrid<S> *)P.desc, P.seg, P.info, (uint8_t *)ctl, nb);
+    rc = check_hip(hipGetLastError(), "batch count kernel launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL((batch_totals_kernel<S>), dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, st, P.desc, L.n,
+                       L.nseg, P.seg, src.gflags, P.totals);
+    rc = check_hip(hipGetLastError(), "batch totals kernel launch");
+    if (rc) return rc;
+    std::vector<diso_totals_t> t(L.n);
+    rc = check_hip(hipMemcpyAsync(t.data(), P.totals, L.n * sizeof(diso_totals_t), hipMemcpyDeviceToHost, st),
+                   "batch totals copy");
+    if (rc) return rc;
+    rc = check_hip(hipStreamSynchronize(st), "stream synchronize");
+    if (rc) return rc;
+    for (int i = 0; i < L.n; ++i) {
+        grids[i].totals = t[i];
+        if (t[i].flags & DISO_TOTAL_SCAN_TIMEOUT) return fail(DISO_E_TIMEOUT, "single-pass scan did not complete");
+    }
+    return DISO_OK;
+}
+
+int64_t runs_of(const Geo &g) { return (g.nseg + g_params.run - 1) / g_params.run; }
+
+template <typename S>
+int batch_emit(const diso_grid_t *grids, const BatchLayout &L, void *ws, hipStream_t st) {
+    Group gr[4];  // MC, MC + deform, DMC, DMC + deform
+    for (int i = 0; i < L.n; ++i) {
+        const diso_grid_t &G = grids[i];
+        if (G.totals.n_verts <= 0) continue;
+        if (G.totals.n_verts > G.vert_capacity || G.totals.n_faces > G.face_capacity)
+            return fail(DISO_E_CAPACITY, "grid %d: output buffers too small", i);
+        if (!G.verts || !G.faces || (G.algo == DISO_DMC && !G.vinfo))
+            return fail(DISO_E_INVALID, "grid %d: null output buffer", i);
+        gr[2 * (G.algo == DISO_DMC) + (G.deform != nullptr)].add(i, runs_of(L.geo[i]));
+    }
+    int rc = upload_batch<S>(ws, L, batch_desc<S>(grids, L), gr, 4, st);
+    if (rc) return rc;
+    const BatchPtrs<S> P = batch_ptrs<S>(ws, L);
+    uint8_t *w = (uint8_t *)ws;
+    for (int k = 0; k < 4; ++k) {
+        const int64_t nb = gr[k].blocks();
+        if (!nb) continue;
+        const dim3 grid((unsigned)nb), block(kBlock);
+        const int32_t *gl = P.gl(L, k, w);
+        const int64_t *rf = P.rf(L, k, w);
+        const int m = gr[k].m();
+        switch (k) {
+            case 0: hipLaunchKernelGGL((emit_mc_batch_kernel<S, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, g_params.run, nb); break;
+            case 1: hipLaunchKernelGGL((emit_mc_batch_kernel<S, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, g_params.run, nb); break;
+            case 2: hipLaunchKernelGGL((emit_dmc_batch_kernel<S, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, g_params.run, nb); break;
+            default: hipLaunchKernelGGL((emit_dmc_batch_kernel<S, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, g_params.run, nb); break;
+        }
+        rc = check_hip(hipGetLastError(), "batch emit kernel launch");
+        if (rc) return rc;
+    }
+    return DISO_OK;
+}
+
+int64_t batch_scratch_bytes(int dtype, const diso_grid_t *grids, int n) {
+    int64_t q = 0;
+    for (int i = 0; i < n; ++i)
+        if (grids[i].algo == DISO_DMC && grids[i].totals.n_faces > 0) q += grids[i].totals.n_faces;
+    return q * 3 * (dtype == DISO_F32 ? 4 : 8);
+}
+
+template <typename S>
+int batch_backward(const diso_grid_t *grids, const BatchLayout &L, void *ws, void *scratch, hipStream_t st) {
+    Group gr[5];  // 0-3: gather, deform x recompute; 4: DMC per-edge gradients
+    for (int i = 0; i < L.n; ++i) {
+        const diso_grid_t &G = grids[i];
+        if (!G.adj_sdf || G.totals.n_verts <= 0) continue;
+        if (G.deform && !G.adj_deform) return fail(DISO_E_INVALID, "grid %d: adj_deform required", i);
+        if (!G.adj_verts) return fail(DISO_E_INVALID, "grid %d: null adj_verts", i);
+        if (G.algo == DISO_DMC) {
+            if (!G.faces || !G.vinfo) return fail(DISO_E_INVALID, "grid %d: DMC backward needs quads and vinfo", i);
+            gr[4].add(i, (G.totals.n_faces + kBlock - 1) / kBlock);
+        }
+        gr[2 * (G.deform != nullptr) + ((G.flags & DISO_FLAG_RECOMPUTE) != 0)].add(i, runs_of(L.geo[i]));
+    }
+    int rc = upload_batch<S>(ws, L, batch_desc<S>(grids, L), gr, 5, st);
+    if (rc) return rc;
+    const BatchPtrs<S> P = batch_ptrs<S>(ws, L);
+    uint8_t *w = (uint8_t *)ws;
+    S *ge = (S *)scratch;
+    if (gr[4].blocks()) {
+        hipLaunchKernelGGL((dmc_edge_grad_batch_kernel<S>), dim3((unsigned)gr[4].blocks()), dim3(kBlock), 0, st,
+                           P.desc, P.gl(L, 4, w), P.rf(L, 4, w), gr[4].m(), ge);
+        rc = check_hip(hipGetLastError(), "batch edge gradient kernel launch");
+        if (rc) return rc;
+    }
+    for (int k = 0; k < 4; ++k) {
+        const int64_t nb = gr[k].blocks();
+        if (!nb) continue;
+        const dim3 grid((unsigned)nb), block(kBlock);
+        const int32_t *gl = P.gl(L, k, w);
+        const int64_t *rf = P.rf(L, k, w);
+        const int m = gr[k].m();
+        switch (k) {
+            case 0: hipLaunchKernelGGL((bwd_batch_kernel<S, false, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, g_params.run, nb); break;
+            case 1: hipLaunchKernelGGL((bwd_batch_kernel<S, false, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, g_params.run, nb); break;
+            case 2: hipLaunchKernelGGL((bwd_batch_kernel<S, true, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, g_params.run, nb); break;
+            default: hipLaunchKernelGGL((bwd_batch_kernel<S, true, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, g_params.run, nb); break;
+        }
+        rc = check_hip(hipGetLastError(), "batch backward kernel launch");
+        if (rc) return rc;
+    }
     return DISO_OK;
 }
 
@@ -1711,6 +2393,86 @@ int diso_split(int dtype, const void *verts, int64_t n_verts, const void *quads,
     hipStream_t st = (hipStream_t)stream;
     return dtype == DISO_F32 ? launch_split<float>(verts, n_verts, quads, n_quads, flags, tris, scratch, st)
                              : launch_split<double>(verts, n_verts, quads, n_quads, flags, tris, scratch, st);
+}
+
+int diso_set_param(int which, int value) {
+    g_err[0] = 0;
+    switch (which) {
+        case DISO_PARAM_RUN:
+            if (value < 1 || value > kMaxRun) return fail(DISO_E_INVALID, "run length must be in [1, %d]", kMaxRun);
+            g_params.run = value;
+            return DISO_OK;
+        case DISO_PARAM_COUNT_BLOCKS_PER_CU:
+            if (value < 1 || value > kCountBlocksPerCU)
+                return fail(DISO_E_INVALID, "count blocks per CU must be in [1, %d]", kCountBlocksPerCU);
+            g_params.count_per_cu = value;
+            return DISO_OK;
+        default:
+            return fail(DISO_E_INVALID, "unknown parameter %d", which);
+    }
+}
+
+int diso_get_param(int which) {
+    switch (which) {
+        case DISO_PARAM_RUN: return g_params.run;
+        case DISO_PARAM_COUNT_BLOCKS_PER_CU: return g_params.count_per_cu;
+        default: return fail(DISO_E_INVALID, "unknown parameter %d", which);
+    }
+}
+
+int64_t diso_batch_seg_bytes(const diso_grid_t *grids, int n) {
+    BatchLayout L;
+    if (!batch_layout(grids, n, &L)) return fail(DISO_E_INVALID, "bad batch");
+    return L.total;
+}
+
+int64_t diso_batch_ctl_bytes(int n) {
+    if (n < 1) return fail(DISO_E_INVALID, "empty batch");
+    return batch_ctl_bytes(n);
+}
+
+int diso_batch_count(int dtype, diso_grid_t *grids, int n, void *seg, int64_t seg_bytes, void *ctl, int64_t ctl_bytes,
+                     void *stream) {
+    g_err[0] = 0;
+    BatchLayout L;
+    int rc = check_batch(dtype, grids, n, &L);
+    if (rc) return rc;
+    if (!seg || !ctl) return fail(DISO_E_INVALID, "null pointer argument");
+    if (seg_bytes < L.total) return fail(DISO_E_CAPACITY, "batch workspace too small");
+    if (ctl_bytes < batch_ctl_bytes(n)) return fail(DISO_E_CAPACITY, "batch ctl workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    return dtype == DISO_F32 ? batch_count<float>(grids, L, seg, ctl, st) : batch_count<double>(grids, L, seg, ctl, st);
+}
+
+int diso_batch_emit(int dtype, const diso_grid_t *grids, int n, void *seg, int64_t seg_bytes, void *stream) {
+    g_err[0] = 0;
+    BatchLayout L;
+    int rc = check_batch(dtype, grids, n, &L);
+    if (rc) return rc;
+    if (!seg) return fail(DISO_E_INVALID, "null pointer argument");
+    if (seg_bytes < L.total) return fail(DISO_E_CAPACITY, "batch workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    return dtype == DISO_F32 ? batch_emit<float>(grids, L, seg, st) : batch_emit<double>(grids, L, seg, st);
+}
+
+int64_t diso_batch_scratch_bytes(int dtype, const diso_grid_t *grids, int n) {
+    if ((dtype != DISO_F32 && dtype != DISO_F64) || !grids || n < 1) return fail(DISO_E_INVALID, "bad batch");
+    return batch_scratch_bytes(dtype, grids, n);
+}
+
+int diso_batch_backward(int dtype, const diso_grid_t *grids, int n, void *seg, int64_t seg_bytes, void *scratch,
+                        int64_t scratch_bytes, void *stream) {
+    g_err[0] = 0;
+    BatchLayout L;
+    int rc = check_batch(dtype, grids, n, &L);
+    if (rc) return rc;
+    if (!seg) return fail(DISO_E_INVALID, "null pointer argument");
+    if (seg_bytes < L.total) return fail(DISO_E_CAPACITY, "batch workspace too small");
+    const int64_t need = batch_scratch_bytes(dtype, grids, n);
+    if (scratch_bytes < need || (need && !scratch)) return fail(DISO_E_CAPACITY, "batch scratch too small");
+    hipStream_t st = (hipStream_t)stream;
+    return dtype == DISO_F32 ? batch_backward<float>(grids, L, seg, scratch, st)
+                             : batch_backward<double>(grids, L, seg, scratch, st);
 }
 
 }  // extern "C"

@@ -65,3 +65,48 @@ def family_grid(kind, n, rng):
     if kind == "noise":
         return rng.uniform(-1.0, 1.0, size=(n, n, n))
     raise ValueError(kind)
+
+
+FAMILIES = ("sphere", "round_box", "blobs", "noise")
+
+
+def uneven_batch_torch(count=64, seed=0, sizes=(32, 48, 64, 96, 128, 192, 256), deform_amp=0.3, device="cuda",
+                       dtype=None):
+    """The uneven batch workload, generated on the device with torch (numpy would take minutes
+    for the 256^3 members): `count` grids with N drawn uniformly from `sizes`, family uniform over
+    sphere (radius U(0.2,0.45)), rounded box (half-extents U(0.2,0.4), radius 0.1), 8 Gaussian
+    blobs (centres U(-0.3,0.3), sigma U(0.05,0.15)) minus 0.5, and i.i.d. U(-1,1); deform
+    U(-deform_amp, deform_amp); algorithms alternate MC, DMC. Values in float64, then cast.
+    Returns [(sdf, deform, algo, family, n)]."""
+    import torch
+
+    dtype = dtype or torch.float32
+    g = torch.Generator(device="cpu").manual_seed(seed)  # sizes, families and shape parameters
+    gd = torch.Generator(device=device).manual_seed(seed + 1)  # the large noise and deform fields
+    u = lambda *shape: torch.rand(*shape, generator=g, dtype=torch.float64)
+    ud = lambda *shape: torch.rand(*shape, generator=gd, dtype=torch.float64, device=device)
+    out = []
+    for i in range(count):
+        n = int(sizes[int(torch.randint(len(sizes), (1,), generator=g))])
+        kind = FAMILIES[int(torch.randint(len(FAMILIES), (1,), generator=g))]
+        a = torch.linspace(-0.5, 0.5, n, dtype=torch.float64, device=device)
+        p = torch.stack(torch.meshgrid(a, a, a, indexing="ij"), -1)
+        if kind == "sphere":
+            f = p.norm(dim=-1) - (0.2 + 0.25 * float(u(1)))
+        elif kind == "round_box":
+            b = (0.2 + 0.2 * u(3)).to(device)
+            q = p.abs() - b + 0.1
+            f = q.clamp_min(0).norm(dim=-1) + q.max(-1)[0].clamp_max(0) - 0.1
+        elif kind == "blobs":
+            c = (u(8, 3) * 0.6 - 0.3).to(device)
+            sg = (0.05 + 0.1 * u(8)).to(device)
+            f = torch.zeros(p.shape[:3], dtype=torch.float64, device=device)
+            for k in range(8):
+                f += torch.exp(-((p - c[k]) ** 2).sum(-1) / (2 * sg[k] ** 2))
+            f -= 0.5
+        else:
+            f = ud(n, n, n) * 2 - 1
+        d = (ud(n, n, n, 3) * 2 - 1) * deform_amp
+        out.append((f.to(dtype).contiguous(), d.to(dtype).contiguous(), "mc" if i % 2 == 0 else "dmc", kind, n))
+        del p
+    return out

@@ -146,6 +146,53 @@ int64_t diso_split_scratch_bytes(int64_t n_quads);
 int diso_split(int dtype, const void *verts, int64_t n_verts, const void *quads, int64_t n_quads, int flags,
                int64_t *tris, int64_t tri_capacity, void *scratch, int64_t scratch_bytes, void *stream);
 
+/* ---- Batches: many grids (uneven dims, MC and DMC, with or without deform) in one launch per
+ * pass. The reference has no batched form: it extracts grid by grid (diso/__init__.py:9-147). Each grid is described by a diso_grid_t; the whole batch shares one
+ * dtype, one workspace ("seg", diso_batch_seg_bytes) and one control block ("ctl",
+ * diso_batch_ctl_bytes: zero-filled once, then reused; calls sharing it must not overlap).
+ *   1. diso_batch_count: classify and count every grid in one launch (one single-pass scan over
+ *      all grids' segments), then fill each grid's `totals` (one host synchronisation).
+ *   2. the caller sizes each grid's outputs from its totals and sets verts / faces / vinfo and
+ *      the capacities, then diso_batch_emit writes every grid's vertices and faces.
+ *   3. later, with the same workspace: set adj_verts / adj_sdf / adj_deform (grids whose adj_sdf is
+ *      NULL are skipped) and call diso_batch_backward.
+ * Per grid, results are identical to diso_count / diso_emit / diso_backward with the same flags.
+ * Each call copies the descriptors to the device and waits for that copy. */
+typedef struct {
+    int32_t algo;  /* DISO_MC or DISO_DMC */
+    int32_t flags; /* DISO_FLAG_* for this grid */
+    int64_t X, Y, Z;
+    double iso;
+    const void *sdf;    /* [X,Y,Z] */
+    const void *deform; /* [X,Y,Z,3] or NULL */
+    void *verts;        /* outputs, rows as in diso_emit */
+    void *faces;
+    void *vinfo;
+    int64_t vert_capacity, face_capacity;
+    const void *adj_verts; /* backward */
+    void *adj_sdf;
+    void *adj_deform;
+    diso_totals_t totals; /* written by diso_batch_count */
+} diso_grid_t;
+
+int64_t diso_batch_seg_bytes(const diso_grid_t *grids, int n);
+int64_t diso_batch_ctl_bytes(int n);
+int diso_batch_count(int dtype, diso_grid_t *grids, int n, void *seg, int64_t seg_bytes, void *ctl,
+                     int64_t ctl_bytes, void *stream);
+int diso_batch_emit(int dtype, const diso_grid_t *grids, int n, void *seg, int64_t seg_bytes, void *stream);
+/* Scratch of diso_batch_backward: the per-edge gradients of all DMC grids. */
+int64_t diso_batch_scratch_bytes(int dtype, const diso_grid_t *grids, int n);
+int diso_batch_backward(int dtype, const diso_grid_t *grids, int n, void *seg, int64_t seg_bytes,
+                        void *scratch, int64_t scratch_bytes, void *stream);
+
+/* Launch parameters, process-wide (set them before launching work; the autotuner in
+ * diso_amd/autotune.py picks them per machine). Results do not depend on them.
+ *  DISO_PARAM_RUN: segments per emission / backward block, 1..64 (default 32).
+ *  DISO_PARAM_COUNT_BLOCKS_PER_CU: resident blocks per CU of the count pass, 1..4 (default 4). */
+enum { DISO_PARAM_RUN = 0, DISO_PARAM_COUNT_BLOCKS_PER_CU = 1 };
+int diso_set_param(int which, int value);
+int diso_get_param(int which);
+
 int diso_read_totals(const void *seg, int64_t seg_bytes, int64_t X, int64_t Y, int64_t Z,
                      int flags, diso_totals_t *totals_host, void *stream);
 

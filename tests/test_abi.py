@@ -110,3 +110,52 @@ def test_split_arguments_checked_before_any_device_work():
     assert L.diso_split_scratch_bytes(1000) >= 1000 and L.diso_split_scratch_bytes(-1) < 0
     with pytest.raises(RuntimeError, match="CUDA"):
         diso_amd.split_quads(torch.zeros(4, 3), torch.zeros(1, 4, dtype=torch.int64))
+
+
+def test_batch_arguments_checked_before_any_device_work():
+    from diso_amd.batch import GridT
+
+    L = _lib.lib()
+    null = ctypes.c_void_p(0)
+    fake = ctypes.c_void_p(0x1000)
+    d = (GridT * 2)()
+    for i, dims in enumerate([(8, 9, 10), (70, 3, 5)]):
+        d[i].algo, d[i].flags = i, _lib.FLAG_VIRTUAL_PAD
+        d[i].X, d[i].Y, d[i].Z = dims
+        d[i].sdf = 0x1000
+    addr = ctypes.addressof(d)
+    nseg = sum((x + 2) * (y + 2) * ((z + 2 + 63) // 64) for x, y, z in [(8, 9, 10), (70, 3, 5)])
+    assert L.diso_batch_seg_bytes(addr, 2) >= (nseg + 2) * 16 + nseg * 128
+    assert L.diso_batch_seg_bytes(addr, 0) < 0 and L.diso_batch_ctl_bytes(0) < 0
+    ctlb = L.diso_batch_ctl_bytes(2)
+    assert ctlb > 0
+    assert L.diso_batch_count(0, addr, 2, fake, 16, fake, ctlb, null) == -3
+    assert L.diso_batch_count(0, addr, 2, fake, 1 << 30, fake, 8, null) == -3
+    assert L.diso_batch_count(4, addr, 2, fake, 1 << 30, fake, ctlb, null) == -1
+    d[1].algo = 9
+    assert L.diso_batch_count(0, addr, 2, fake, 1 << 30, fake, ctlb, null) == -1
+    assert b"grid 1" in L.diso_last_error()
+    d[1].algo = 1
+    d[0].sdf = None
+    assert L.diso_batch_emit(0, addr, 2, fake, 1 << 30, null) == -1
+    d[0].sdf = 0x1000
+    d[1].totals.n_faces = 10
+    assert L.diso_batch_scratch_bytes(0, addr, 2) == 120 and L.diso_batch_scratch_bytes(1, addr, 2) == 240
+    rc = L.diso_batch_backward(0, addr, 2, fake, 1 << 30, null, 0, null)
+    assert rc == -3 and b"scratch" in L.diso_last_error()
+
+
+def test_launch_parameters():
+    from diso_amd import autotune
+
+    L = _lib.lib()
+    before = autotune.current()
+    try:
+        assert L.diso_set_param(_lib.PARAM_RUN, 0) == -1 and L.diso_set_param(_lib.PARAM_RUN, 65) == -1
+        assert L.diso_set_param(_lib.PARAM_COUNT_BLOCKS_PER_CU, 5) == -1 and L.diso_set_param(9, 1) == -1
+        autotune.apply({"run": 16, "count_blocks_per_cu": 2})
+        assert autotune.current() == {"run": 16, "count_blocks_per_cu": 2}
+        with pytest.raises(RuntimeError):
+            autotune.apply({"run": 100, "count_blocks_per_cu": 2})
+    finally:
+        autotune.apply(before)

@@ -234,3 +234,28 @@ def test_extracted_meshes_are_watertight_and_export(tmp_path):
         mesh.write_ply(tmp_path / "m.ply", v, f)
         v2, f2 = mesh.read_ply(tmp_path / "m.ply")
         assert np.array_equal(v2, v.cpu().numpy()) and np.array_equal(f2, f.cpu().numpy())
+
+
+def test_results_do_not_depend_on_launch_parameters():
+    from diso_amd import autotune
+
+    sdf_np, def_np = inputs((37, 50, 150), "noisy_box", 9, np.float32, True)
+    before = autotune.current()
+    outs = []
+    try:
+        for run, cpc in ((1, 1), (7, 3), (32, 4), (64, 2)):
+            print(f"case launch run={run} count_blocks_per_cu={cpc}", flush=True)
+            autotune.apply({"run": run, "count_blocks_per_cu": cpc})
+            res = []
+            for mod in (diso_amd.DiffMC(), diso_amd.DiffDMC()):
+                sdf = torch.tensor(sdf_np, device=DEV, requires_grad=True)
+                deform = torch.tensor(def_np, device=DEV, requires_grad=True)
+                v, f = mod(sdf, deform)
+                (v * v).sum().backward()
+                res += [v.detach().cpu().numpy(), f.cpu().numpy(), sdf.grad.cpu().numpy(), deform.grad.cpu().numpy()]
+            outs.append(res)
+    finally:
+        autotune.apply(before)
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
