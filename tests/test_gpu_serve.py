@@ -1,0 +1,70 @@
+"""Serving loop (needs a GPU): requests submitted from several threads and batched by two
+workers give the same results, bit for bit, as each request extracted alone; a tight memory
+budget serialises the work without changing results."""
+import threading
+
+import numpy as np
+import pytest
+import torch
+
+import diso_amd
+from diso_amd.serve import Request, Server
+from tests.helpers import inputs
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+MIX = [((24, 24, 24), "noisy_box", "mc", True, True), ((33, 20, 70), "noise", "dmc", True, True),
+       ((40, 9, 130), "blobs", "mc", False, False), ((64, 64, 64), "sphere", "dmc", False, True),
+       ((5, 3, 66), "noise", "dmc", True, False), ((31, 47, 29), "sphere", "mc", True, True),
+       ((48, 40, 72), "noise", "dmc", True, True), ((17, 33, 70), "noise", "mc", False, True),
+       ((8, 8, 8), "sphere", "mc", False, False), ((50, 50, 50), "noisy_box", "dmc", True, True)]
+
+
+def _requests(seed):
+    reqs, alone = [], []
+    for i, (dims, kind, algo, with_deform, backward) in enumerate(MIX):
+        sdf_np, def_np = inputs(dims, kind, seed + i, np.float32, with_deform)
+        sdf = torch.tensor(sdf_np, device=DEV)
+        deform = None if def_np is None else torch.tensor(def_np, device=DEV)
+        gen = torch.Generator(device=DEV).manual_seed(seed + i)
+        up = (lambda n, g=gen: torch.randn((n, 3), generator=g, device=DEV)) if backward else None
+        reqs.append(Request(sdf, algo, deform, upstream=up))
+        alone.append((sdf, deform, algo, backward, seed + i))
+    return reqs, alone
+
+
+def _alone(sdf, deform, algo, backward, seed):
+    s = sdf.detach().requires_grad_(backward)
+    d = None if deform is None else deform.detach().requires_grad_(backward)
+    mod = diso_amd.DiffMC() if algo == "mc" else diso_amd.DiffDMC()
+    v, f = mod(s, d, return_quads=True) if algo == "dmc" else mod(s, d)
+    if not backward or not v.shape[0]:
+        return v.detach(), f, None, None
+    g = torch.randn((v.shape[0], 3), generator=torch.Generator(device=DEV).manual_seed(seed), device=DEV)
+    gs, *gd = torch.autograd.grad([v], [s] + ([d] if d is not None else []), grad_outputs=[g])
+    return v.detach(), f, gs, gd[0] if gd else None
+
+
+@pytest.mark.parametrize("budget", [200 << 30, 4 << 20])
+def test_served_results_equal_alone(budget):
+    print(f"case serve budget={budget}", flush=True)
+    reqs, alone = _requests(100)
+    with Server(budget_bytes=budget, workers=2, max_batch=4) as srv:
+        threads = [threading.Thread(target=lambda rs=reqs[k::3]: [srv.submit(r) for r in rs]) for k in range(3)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        results = [r.future.result(timeout=120) for r in reqs]
+        assert srv.batches >= 1 and len(srv.latencies) == len(reqs)
+        if budget < (64 << 20):
+            assert srv.peak_reserved <= max(budget, max(r.estimate() for r in reqs))
+        assert srv.latency_ms(50) is not None and srv.latency_ms(99) >= srv.latency_ms(50)
+    for res, a in zip(results, alone):
+        v, f, gs, gd = _alone(*a)
+        assert torch.equal(res[0], v) and torch.equal(res[1], f)
+        if gs is not None:
+            assert torch.equal(res[2], gs)
+            if gd is not None:
+                assert torch.equal(res[3], gd)
