@@ -19,8 +19,9 @@ from torch import nn
 from . import _lib
 from .batch import BatchExtractor
 from .engine import Extractor, split_quads
+from .fixed import FixedExtractor
 
-__all__ = ["DiffMC", "DiffDMC", "BatchExtractor", "split_quads"]
+__all__ = ["DiffMC", "DiffDMC", "BatchExtractor", "FixedExtractor", "split_quads"]
 __version__ = "0.1.0"
 
 

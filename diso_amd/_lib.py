@@ -33,6 +33,7 @@ FLAG_RECOMPUTE = 1 << 6
 TOTAL_HAS_BELOW = 1 << 0
 TOTAL_HAS_ABOVE = 1 << 1
 TOTAL_HAS_NAN = 1 << 2
+TOTAL_SCAN_TIMEOUT = 1 << 3
 PARAM_RUN = 0
 PARAM_COUNT_BLOCKS_PER_CU = 1
 
@@ -40,7 +41,7 @@ EXPORTS = [
     "diso_seg_bytes", "diso_ctl_bytes", "diso_scratch_bytes", "diso_count", "diso_emit", "diso_backward",
     "diso_read_totals", "diso_last_error", "diso_version", "diso_split_scratch_bytes", "diso_split",
     "diso_batch_seg_bytes", "diso_batch_ctl_bytes", "diso_batch_count", "diso_batch_emit", "diso_batch_scratch_bytes",
-    "diso_batch_backward", "diso_set_param", "diso_get_param",
+    "diso_batch_backward", "diso_set_param", "diso_get_param", "diso_totals_offset",
 ]
 
 
@@ -80,6 +81,8 @@ def lib():
         L.diso_seg_bytes.argtypes = [i64, i64, i64, i32]
         L.diso_ctl_bytes.restype = i64
         L.diso_ctl_bytes.argtypes = [i64, i64, i64, i32]
+        L.diso_totals_offset.restype = i64
+        L.diso_totals_offset.argtypes = [i64, i64, i64, i32]
         L.diso_count.restype = i32
         L.diso_count.argtypes = [i32, i32, vp, i64, i64, i64, dbl, i32, vp, i64, vp, i64,
                                  ctypes.POINTER(Totals), vp]

@@ -810,41 +810,65 @@ struct McEmitLds {
     uint8_t vert[kWave * 3];
 };
 
-// One segment that has triangles (hence cells, so rows +x, +y, +x+y exist).
+// What an MC segment reads from memory before any work (loaded one segment ahead): the
+// classification words of rows 00, 10, 01, 11 at z and of rows 00, 10 at z + 1, and the bases.
+struct McSegIn {
+    int64_t s;
+    SegPos sp;
+    uint32_t w00, w10, w01, w11, n00, n10;
+    int64_t vabs, fabs, v10, v01, v11;
+};
+
+template <typename S>
+__device__ __forceinline__ McSegIn mc_load(const In<S> &in, const SegCtx &sc, int64_t s, SegPos sp, int lane) {
+    const Geo &g = in.geo;
+    const SegBase *__restrict__ seg = sc.seg;
+    const uint16_t *info = sc.info;
+    McSegIn d;
+    d.s = s;
+    d.sp = sp;
+    // This row has cells, so rows +x, +y, +x+y exist; +z words are read within the row only.
+    const int px = sp.px, py = sp.py, chunk = sp.chunk, pz = chunk * kWave + lane;
+    const int64_t s10 = seg_index(g, px + 1, py, chunk), s01 = seg_index(g, px, py + 1, chunk),
+                  s11 = seg_index(g, px + 1, py + 1, chunk);
+    d.vabs = seg[s].v;
+    d.fabs = seg[s].f;
+    d.v10 = seg[s10].v;
+    d.v01 = seg[s01].v;
+    d.v11 = seg[s11].v;
+    d.w00 = info[s * kWave + lane];
+    d.w10 = info[s10 * kWave + lane];
+    d.w01 = info[s01 * kWave + lane];
+    d.w11 = info[s11 * kWave + lane];
+    const bool zn = pz + 1 < g.g2;
+    d.n00 = zn ? info[s * kWave + lane + 1] : 0u;
+    d.n10 = zn ? info[s10 * kWave + lane + 1] : 0u;
+    return d;
+}
+
+// One segment that has triangles.
 template <typename S, bool DEF>
-__device__ __forceinline__ void emit_mc_segment(const In<S> &in, const SegCtx &sc, int64_t s, SegPos sp,
+__device__ __forceinline__ void emit_mc_segment(const In<S> &in, const SegCtx &sc, const McSegIn &d,
                                                 S *__restrict__ verts, void *__restrict__ faces, const OutCfg &out,
                                                 const Norm<S> &nm, const uint64_t *s_case, McEmitLds &L,
                                                 int lane) {
     wave_lds_sync();  // the previous segment's LDS reads are done
-    const Geo &g = in.geo;
-    const SegBase *__restrict__ seg = sc.seg;
-    const int64_t tbase = seg[s].f - sc.f0;
-    const int px = sp.px, py = sp.py, chunk = sp.chunk;
-    const int z0 = chunk * kWave, pz = z0 + lane;
-    const int64_t vabs = seg[s].v, vbase = vabs - sc.v0;
-
-    // This row has cells, so rows +x, +y, +x+y exist; +z words are read within the row only.
-    const uint16_t *info = sc.info;
-    const int64_t s10 = seg_index(g, px + 1, py, chunk), s01 = seg_index(g, px, py + 1, chunk),
-                  s11 = seg_index(g, px + 1, py + 1, chunk);
-    const uint32_t w00 = info[s * kWave + lane];
-    const int m00 = info_mask(w00);
-    const int m10 = info_mask(info[s10 * kWave + lane]);
-    const int m01 = info_mask(info[s01 * kWave + lane]);
-    const int m11 = info_mask(info[s11 * kWave + lane]);
-    const bool zn = pz + 1 < g.g2;
-    const int c00n = zn ? info_mask(info[s * kWave + lane + 1]) & 1 : 0;
-    const int c10n = zn ? info_mask(info[s10 * kWave + lane + 1]) & 1 : 0;
+    const int64_t tbase = d.fabs - sc.f0;
+    const int px = d.sp.px, py = d.sp.py;
+    const int z0 = d.sp.chunk * kWave;
+    const int64_t vabs = d.vabs, vbase = vabs - sc.v0;
+    const uint32_t w00 = d.w00;
+    const int m00 = info_mask(w00), m10 = info_mask(d.w10), m01 = info_mask(d.w01), m11 = info_mask(d.w11);
+    const int c00n = info_mask(d.n00) & 1, c10n = info_mask(d.n10) & 1;
     const uint64_t cw = s_case[info_code(w00)];  // Bourke corner order (src/cumc.cu:107)
     const int ntri = DISO_MC_NTRI(cw);
 
     int tv, tt, t;
     const int p00 = wave_excl_small(popc3(m00), &tv);
     const int pt = wave_excl_small(ntri, &tt);
-    const int32_t r10 = (int32_t)(seg[s10].v - vabs) + wave_excl_small(popc3(m10), &t);
-    const int32_t r01 = (int32_t)(seg[s01].v - vabs) + wave_excl_small(popc3(m01), &t);
-    const int32_t r11 = (int32_t)(seg[s11].v - vabs) + wave_excl_small(popc3(m11), &t);
+    const int32_t r10 = (int32_t)(d.v10 - vabs) + wave_excl_small(popc3(m10), &t);
+    const int32_t r01 = (int32_t)(d.v01 - vabs) + wave_excl_small(popc3(m01), &t);
+    const int32_t r11 = (int32_t)(d.v11 - vabs) + wave_excl_small(popc3(m11), &t);
 
     {
         int r = p00;
@@ -887,12 +911,23 @@ __global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase
     // segments of the run with triangles (hence owned crossings), dealt round-robin to the waves
     uint64_t act = run_active(seg, lo, hi, lane);
     const SegPos lp = seg_pos(g, lo + lane);
+    // this wave's segments: every kWavesPerBlock-th active one; loads run one segment ahead
+    uint64_t mine = 0;
     for (int k = 0; act; ++k, act &= act - 1)
-        if ((k & (kWavesPerBlock - 1)) == wave) {
-            const int i = __builtin_ctzll(act);
-            emit_mc_segment<S, DEF>(in, sc, lo + i, pos_of_lane(lp, i), verts, faces, out, nm, s_case, s_lds[wave],
-                                    lane);
+        if ((k & (kWavesPerBlock - 1)) == wave) mine |= act & (0ull - act);
+    if (mine) {
+        int i = __builtin_ctzll(mine);
+        McSegIn nxt = mc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
+        while (mine) {
+            const McSegIn cur = nxt;
+            mine &= mine - 1;
+            if (mine) {
+                i = __builtin_ctzll(mine);
+                nxt = mc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
+            }
+            emit_mc_segment<S, DEF>(in, sc, cur, verts, faces, out, nm, s_case, s_lds[wave], lane);
         }
+    }
 }
 
 // DMC emission. Patch-vertex item = (lane, patch): the mean of the MC vertices on the patch's
@@ -932,7 +967,8 @@ constexpr int kEdgeSlots = 320;  // MC vertices kept per segment; beyond, patche
 template <typename S>
 struct DmcEmitLds {
     uint8_t code[8][kWave];    // codes of the cells (-i,-j,-k) around the lane, ci = 4i + 2j + k
-    int32_t base[8][kWave];    // their first DMC vertex, relative to the segment's first
+    int16_t pre[8][kWave];     // their first DMC vertex, relative to the first of row (-i,-j)
+    int32_t rowb[4];           // first DMC vertex of row (-i,-j), relative to the segment's first
     uint8_t patch[kWave * 4];  // patch items: lane * 4 + patch
     uint16_t quad[kWave * 3];  // quad items: lane * 8 + axis * 2 + leaving
     uint8_t ex[8][kWave];      // crossing edges of each kind at the lanes before
@@ -962,73 +998,118 @@ constexpr PatchEdgeTable make_patch_edges() {
 }
 __constant__ PatchEdgeTable kDmcPatchEdges = make_patch_edges();
 
-// Block-wide DMC tables: case words, patch edge masks, the quad stencil and the reciprocals 1/n
-// of the possible patch sizes.
+// Quad stencil of each edge type as one word: cell index ci of corner j in bits 3j..3j+2, edge id
+// in bits 12+4j..15+4j.
+struct StencilTable {
+    uint32_t w[6];
+};
+constexpr StencilTable make_stencil() {
+    StencilTable t{};
+    for (int ty = 0; ty < 6; ++ty)
+        for (int j = 0; j < 4; ++j)
+            t.w[ty] |= ((uint32_t)stencil_ci(ty, j) << (3 * j)) | ((uint32_t)stencil_eid(ty, j) << (12 + 4 * j));
+    return t;
+}
+__constant__ StencilTable kStencil = make_stencil();
+
+// 1/n for the possible patch sizes (n = 1..7), correctly rounded at compile time.
 template <typename S>
-struct DmcTables {
-    uint64_t cases[256];
-    uint64_t pedges[256];
-    S rcp[8];
-    uint8_t st_ci[6][4], st_eid[6][4];
+struct RcpTable {
+    S r[8];
+};
+template <typename S>
+constexpr RcpTable<S> make_rcp() {
+    RcpTable<S> t{};
+    for (int n = 1; n < 8; ++n) t.r[n] = S(1) / S(n);
+    return t;
+}
+__constant__ RcpTable<float> kRcpF = make_rcp<float>();
+__constant__ RcpTable<double> kRcpD = make_rcp<double>();
+template <typename S>
+__device__ __forceinline__ S rcp_small(int n);
+template <>
+__device__ __forceinline__ float rcp_small<float>(int n) { return kRcpF.r[n]; }
+template <>
+__device__ __forceinline__ double rcp_small<double>(int n) { return kRcpD.r[n]; }
+
+// What a DMC segment reads from memory before any work: its classification words (own row,
+// rows -x, -y, -x-y at z and z - 1, rows +x, +y, +x+y at z, and the halo position z0 + 64) and the
+// bases. Loaded one segment ahead, so the loads of the next segment overlap this one's work.
+struct DmcSegIn {
+    int64_t s;
+    SegPos sp;
+    uint32_t w0, w10, w01, w11, h00, h10, h01;
+    uint32_t cur[4], prev[4];  // codes of cells (-i,-j,0) and (-i,-j,-1), index 2i + j
+    int64_t vabs, pabs, rowf[4];
 };
 
 template <typename S>
-__device__ __forceinline__ void load_dmc_tables(DmcTables<S> &T, int tid) {
-    for (int i = tid; i < 256; i += kBlock) {
-        T.cases[i] = kDmcCase[i];
-        T.pedges[i] = kDmcPatchEdges.m[i];
-    }
-    if (tid < 8) T.rcp[tid] = tid ? S(1) / S(tid) : S(0);
-    if (tid < 24) {
-        const int ty = tid / 4, k = tid % 4;
-        T.st_ci[ty][k] = (uint8_t)stencil_ci(ty, k);
-        T.st_eid[ty][k] = (uint8_t)stencil_eid(ty, k);
-    }
-}
-
-// One segment that has patches (hence cells, so rows +x, +y exist).
-template <typename S, bool DEF>
-__device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &sc, int64_t s, SegPos sp,
-                                                 S *__restrict__ verts, void *__restrict__ faces,
-                                                 uint8_t *__restrict__ vinfo, const OutCfg &out, const Norm<S> &nm,
-                                                 const DmcTables<S> &T, DmcEmitLds<S> &L, int lane) {
-    wave_lds_sync();  // the previous segment's LDS reads are done
+__device__ __forceinline__ DmcSegIn dmc_load(const In<S> &in, const SegCtx &sc, int64_t s, SegPos sp, int lane) {
     const Geo &g = in.geo;
     const SegBase *__restrict__ seg = sc.seg;
-    const int64_t pabs = seg[s].f, pbase = pabs - sc.f0;
-    const int px = sp.px, py = sp.py, chunk = sp.chunk;
-    const int z0 = chunk * kWave, pz = z0 + lane;
-    const int64_t qbase = seg[s].v - sc.v0;
     const uint16_t *info = sc.info;
-    const uint32_t w0 = info[s * kWave + lane];
-    // Crossing masks of the rows the cells' edges start from: 10, 01, 11 (this z) and the halo.
+    DmcSegIn d;
+    d.s = s;
+    d.sp = sp;
+    const int px = sp.px, py = sp.py, chunk = sp.chunk, pz = chunk * kWave + lane;
+    d.vabs = seg[s].v;
+    d.pabs = seg[s].f;
     const int64_t s10 = seg_index(g, px + 1, py, chunk), s01 = seg_index(g, px, py + 1, chunk),
                   s11 = seg_index(g, px + 1, py + 1, chunk);
-    const int m00 = info_mask(w0), m10 = info_mask(info[s10 * kWave + lane]),
-              m01 = info_mask(info[s01 * kWave + lane]), m11 = info_mask(info[s11 * kWave + lane]);
-    int h00 = 0, h10 = 0, h01 = 0;
+    d.w0 = info[s * kWave + lane];
+    d.w10 = info[s10 * kWave + lane];
+    d.w01 = info[s01 * kWave + lane];
+    d.w11 = info[s11 * kWave + lane];
+    d.h00 = d.h10 = d.h01 = 0;
     if (lane == kWave - 1 && pz + 1 < g.g2) {  // position z0 + 64, in the row's next segment
-        h00 = info_mask(info[s * kWave + lane + 1]);
-        h10 = info_mask(info[s10 * kWave + lane + 1]);
-        h01 = info_mask(info[s01 * kWave + lane + 1]);
+        d.h00 = info[s * kWave + lane + 1];
+        d.h10 = info[s10 * kWave + lane + 1];
+        d.h01 = info[s01 * kWave + lane + 1];
     }
-    // Codes and first vertices of the cells (-i, -j, 0) and (-i, -j, -1).
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const bool row = px - i >= 0 && py - j >= 0;  // wave-uniform
-            const int64_t r = row ? seg_index(g, px - i, py - j, chunk) : 0;
-            const int cur = (i || j) ? (row ? info_code(info[r * kWave + lane]) : 0) : info_code(w0);
-            const int prev = (row && !(i && j) && pz >= 1) ? info_code(info[r * kWave + lane - 1]) : 0;
-            int t;
-            const int32_t b = (int32_t)((row ? seg[r].f : pabs) - pabs) + wave_excl_small(DISO_DMC_NPATCH(T.cases[cur]), &t);
-            L.code[4 * i + 2 * j][lane] = (uint8_t)cur;
-            L.base[4 * i + 2 * j][lane] = b;
-            L.code[4 * i + 2 * j + 1][lane] = (uint8_t)prev;
-            L.base[4 * i + 2 * j + 1][lane] = b - DISO_DMC_NPATCH(T.cases[prev]);
+            const int64_t r = row ? seg_index(g, px - i, py - j, chunk) : s;
+            d.rowf[2 * i + j] = row ? seg[r].f : d.pabs;
+            d.cur[2 * i + j] = (i || j) ? (row ? info[r * kWave + lane] : 0u) : d.w0;
+            d.prev[2 * i + j] = (row && !(i && j) && pz >= 1) ? info[r * kWave + lane - 1] : 0u;
         }
-    const int np = DISO_DMC_NPATCH(T.cases[info_code(w0)]);
+    return d;
+}
+
+// One segment that has patches (hence cells, so rows +x, +y exist).
+template <typename S, bool DEF>
+__device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &sc, const DmcSegIn &d,
+                                                 S *__restrict__ verts, void *__restrict__ faces,
+                                                 uint8_t *__restrict__ vinfo, const OutCfg &out, const Norm<S> &nm,
+                                                 DmcEmitLds<S> &L, int lane) {
+    wave_lds_sync();  // the previous segment's LDS reads are done
+    const Geo &g = in.geo;
+    const int64_t pabs = d.pabs, pbase = pabs - sc.f0;
+    const int px = d.sp.px, py = d.sp.py;
+    const int z0 = d.sp.chunk * kWave;
+    const int64_t qbase = d.vabs - sc.v0;
+    const uint32_t w0 = d.w0;
+    // Crossing masks of the rows the cells' edges start from: 10, 01, 11 (this z) and the halo.
+    const int m00 = info_mask(w0), m10 = info_mask(d.w10), m01 = info_mask(d.w01), m11 = info_mask(d.w11);
+    const int h00 = info_mask(d.h00), h10 = info_mask(d.h10), h01 = info_mask(d.h01);
+    // Codes and first vertices of the cells (-i, -j, 0) and (-i, -j, -1).
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int cur = info_code(d.cur[2 * i + j]), prev = info_code(d.prev[2 * i + j]);
+            int t;
+            const int b = wave_excl_small(DISO_DMC_NPATCH(kDmcCase[cur]), &t);
+            if (lane == 0) L.rowb[2 * i + j] = (int32_t)(d.rowf[2 * i + j] - pabs);
+            L.code[4 * i + 2 * j][lane] = (uint8_t)cur;
+            L.pre[4 * i + 2 * j][lane] = (int16_t)b;
+            L.code[4 * i + 2 * j + 1][lane] = (uint8_t)prev;
+            L.pre[4 * i + 2 * j + 1][lane] = (int16_t)(b - DISO_DMC_NPATCH(kDmcCase[prev]));
+        }
+    const int np = DISO_DMC_NPATCH(kDmcCase[info_code(w0)]);
     int tq, tp;
     const int pq = wave_excl_small(popc3(m00), &tq);
     const int pp = wave_excl_small(np, &tp);
@@ -1085,7 +1166,7 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
     constexpr uint64_t kKinds = edge_kind_table();
     for (int k = lane; k < tp; k += kWave) {
         const int it = L.patch[k], src = it >> 2, pk = it & 3;
-        unsigned em = (unsigned)(T.pedges[L.code[0][src]] >> (16 * pk)) & 0xfffu;
+        unsigned em = (unsigned)(kDmcPatchEdges.m[L.code[0][src]] >> (16 * pk)) & 0xfffu;
         const int num = __popc(em);
         S sx = S(0), sy = S(0), sz = S(0);
         if (slots) {
@@ -1113,7 +1194,7 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
                 sz += p.z;
             }
         }
-        const S r = T.rcp[num];
+        const S r = rcp_small<S>(num);
         store_vertex(verts, pbase + k, out.vcap, epilogue(V3<S>{sx * r, sy * r, sz * r}, out.flags, nm));
         if (pbase + k < out.vcap) vinfo[pbase + k] = (uint8_t)(num | (pk << 3));
     }
@@ -1121,12 +1202,13 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
         const int it = L.quad[k], src = it >> 3, ty = ((it >> 1) & 3) + 3 * (it & 1);
         const int64_t q = qbase + k;
         if (q >= out.fcap) continue;
+        const uint32_t stw = kStencil.w[ty];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int ci = T.st_ci[ty][j], eid = T.st_eid[ty][j];
-            const int p = DISO_DMC_PATCH_OF(T.cases[L.code[ci][src]], eid);
+            const int ci = (stw >> (3 * j)) & 7, eid = (stw >> (12 + 4 * j)) & 15;
+            const int p = DISO_DMC_PATCH_OF(kDmcCase[L.code[ci][src]], eid);
             // dmcEdgeOffset is -1 for a non-crossing edge (never the case on a padded grid)
-            store_index(faces, 4 * q + j, pbase + L.base[ci][src] + (p == 7 ? -1 : p), out.flags);
+            store_index(faces, 4 * q + j, pbase + L.rowb[ci >> 1] + L.pre[ci][src] + (p == 7 ? -1 : p), out.flags);
         }
     }
 }
@@ -1136,11 +1218,8 @@ __global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBas
                                                           S *__restrict__ verts, void *__restrict__ faces,
                                                           uint8_t *__restrict__ vinfo, OutCfg out, Norm<S> nm,
                                                           int run, int64_t nblocks) {
-    __shared__ DmcTables<S> T;
     __shared__ DmcEmitLds<S> s_lds[kWavesPerBlock];
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
-    load_dmc_tables(T, tid);
-    __syncthreads();
     const Geo &g = in.geo;
     const SegCtx sc{seg, info_of(seg, g), 0, 0};
     const SegBase *tot = seg + g.nseg;
@@ -1149,12 +1228,23 @@ __global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBas
     // segments of the run with patches (hence owned crossings), dealt round-robin to the waves
     uint64_t act = run_active(seg, lo, hi, lane);
     const SegPos lp = seg_pos(g, lo + lane);
+    // this wave's segments: every kWavesPerBlock-th active one; loads run one segment ahead
+    uint64_t mine = 0;
     for (int k = 0; act; ++k, act &= act - 1)
-        if ((k & (kWavesPerBlock - 1)) == wave) {
-            const int i = __builtin_ctzll(act);
-            emit_dmc_segment<S, DEF>(in, sc, lo + i, pos_of_lane(lp, i), verts, faces, vinfo, out, nm, T, s_lds[wave],
-                                     lane);
+        if ((k & (kWavesPerBlock - 1)) == wave) mine |= act & (0ull - act);
+    if (mine) {
+        int i = __builtin_ctzll(mine);
+        DmcSegIn nxt = dmc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
+        while (mine) {
+            const DmcSegIn cur = nxt;
+            mine &= mine - 1;
+            if (mine) {
+                i = __builtin_ctzll(mine);
+                nxt = dmc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
+            }
+            emit_dmc_segment<S, DEF>(in, sc, cur, verts, faces, vinfo, out, nm, s_lds[wave], lane);
         }
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1271,13 +1361,10 @@ __device__ __forceinline__ void store_grad(const In<S> &in, S *__restrict__ adj_
     }
 }
 
-// Per-wave LDS of the gather: each vertex's sample and deform; the vertex id of each incident
-// edge relative to the segment's first vertex; the item list (lane * 8 + slot) and one round of
-// item results.
+// Per-wave LDS of the gather: the vertex id of each incident edge relative to the segment's first
+// vertex; the item list (lane * 8 + slot) and one round of item results.
 template <typename S>
 struct BwdLds {
-    S c[kWave];
-    S fc[3][kWave];
     int32_t id[6][kWave];
     uint16_t item[kWave * 6];
     Grad<S> res[kWave];
@@ -1297,25 +1384,53 @@ __device__ __forceinline__ bool seg_touched(const SegBase *__restrict__ seg, con
            (sp.chunk > 0 && seg_owns_crossing(seg, s - 1));
 }
 
+// What a gather segment reads before any work: the vertex bases of the segment and of rows -x, -y
+// (the segment's own base where the row does not exist) and, unless the masks are recomputed from
+// the samples, the classification words of the vertex and of its -x, -y, -z neighbours. Loaded one
+// segment ahead, so these loads overlap the previous segment's work.
+struct BwdSegIn {
+    int64_t s;
+    SegPos sp;
+    int64_t v, vm0, v0m;
+    uint32_t w00, wm0, w0m, wzm;
+};
+
+template <bool RECOMP>
+__device__ __forceinline__ BwdSegIn bwd_load(const Geo &g, const SegCtx &sc, int64_t s, SegPos sp, int lane) {
+    const SegBase *__restrict__ seg = sc.seg;
+    BwdSegIn d;
+    d.s = s;
+    d.sp = sp;
+    const int64_t sm0 = sp.px > 0 ? seg_index(g, sp.px - 1, sp.py, sp.chunk) : -1;
+    const int64_t s0m = sp.py > 0 ? seg_index(g, sp.px, sp.py - 1, sp.chunk) : -1;
+    d.v = seg[s].v;
+    d.vm0 = sm0 >= 0 ? seg[sm0].v : d.v;
+    d.v0m = s0m >= 0 ? seg[s0m].v : d.v;
+    d.w00 = d.wm0 = d.w0m = d.wzm = 0;
+    if constexpr (!RECOMP) {
+        const uint16_t *info = sc.info;
+        d.w00 = info[s * kWave + lane];
+        if (sm0 >= 0) d.wm0 = info[sm0 * kWave + lane];
+        if (s0m >= 0) d.w0m = info[s0m * kWave + lane];
+        if (sp.chunk * kWave + lane > 0) d.wzm = info[s * kWave + lane - 1];
+    }
+    return d;
+}
+
 // Gather of one touched segment.
 template <typename S, bool DEF, bool RECOMP>
-__device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, int64_t s, SegPos sp,
+__device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, const BwdSegIn &sg,
                                             const AdjIn<S> &adj, S *__restrict__ adj_sdf,
                                             S *__restrict__ adj_deform, BwdLds<S> &L, int lane) {
     wave_lds_sync();  // the previous segment's LDS reads are done
-    const Geo &g = in.geo;
-    const SegBase *__restrict__ seg = sc.seg;
-    const int px = sp.px, py = sp.py, chunk = sp.chunk;
+    const int px = sg.sp.px, py = sg.sp.py, chunk = sg.sp.chunk;
     const int z0 = chunk * kWave, pz = z0 + lane;
     const S iso = in.iso;
-    const int64_t sm0 = px > 0 ? seg_index(g, px - 1, py, chunk) : -1;
-    const int64_t s0m = py > 0 ? seg_index(g, px, py - 1, chunk) : -1;
     Grad<S> acc = {S(0), S(0), S(0), S(0)};
-    const int64_t vabs = seg[s].v, vbase = vabs - sc.v0;
+    const int64_t vabs = sg.v, vbase = vabs - sc.v0;
     // Crossing masks of the edges owned by the vertex and by its -x, -y, -z neighbours: from the
     // classification words, or (RECOMP) from the samples of the vertex's neighbourhood.
     int m00, mm0, m0m, mzm;
-    S c = S(0);
     if constexpr (RECOMP) {
         Win<S, -1, 3, -1, 3, -1, 3> w(in, px, py, pz, z0);
         w.template load_box<0, 1, 0, 1, -1, 3>(in);
@@ -1324,34 +1439,26 @@ __device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, i
         w.template load_box<1, 1, -1, 2, 0, 1>(in);
         w.template load_box<0, 1, -1, 1, 0, 2>(in);
         w.template load_box<0, 1, 1, 1, 0, 1>(in);
-        c = w(0, 0, 0);
+        const S c = w(0, 0, 0);
         m00 = is_anchor(in, px, py, pz) ? own_mask(c, w(1, 0, 0), w(0, 1, 0), w(0, 0, 1), iso) : 0;
         mm0 = is_anchor(in, px - 1, py, pz) ? own_mask(w(-1, 0, 0), c, w(-1, 1, 0), w(-1, 0, 1), iso) : 0;
         m0m = is_anchor(in, px, py - 1, pz) ? own_mask(w(0, -1, 0), w(1, -1, 0), c, w(0, -1, 1), iso) : 0;
         mzm = is_anchor(in, px, py, pz - 1) ? crosses(w(0, 0, -1), c, iso) << 2 : 0;
     } else {
-        const uint16_t *info = sc.info;
-        m00 = info_mask(info[s * kWave + lane]);
-        mm0 = sm0 >= 0 ? info_mask(info[sm0 * kWave + lane]) : 0;
-        m0m = s0m >= 0 ? info_mask(info[s0m * kWave + lane]) : 0;
-        mzm = pz > 0 ? info_mask(info[s * kWave + lane - 1]) : 0;
+        m00 = info_mask(sg.w00);
+        mm0 = info_mask(sg.wm0);
+        m0m = info_mask(sg.w0m);
+        mzm = info_mask(sg.wzm);
     }
     // slots 0-2: edges the vertex owns; 3-5: the x edge of -x, the y edge of -y, the z edge of -z
     const int inc_m = (mm0 & 1) | (m0m & 2) | (mzm & 4);
     const int mask = in_data(in, px, py, pz) ? (m00 | (inc_m << 3)) : 0;
     int t, total;
     const int32_t r00 = wave_excl_small(popc3(m00), &t);
-    const int32_t rm0 = (int32_t)((sm0 >= 0 ? seg[sm0].v : vabs) - vabs) + wave_excl_small(popc3(mm0), &t);
-    const int32_t r0m = (int32_t)((s0m >= 0 ? seg[s0m].v : vabs) - vabs) + wave_excl_small(popc3(m0m), &t);
+    const int32_t rm0 = (int32_t)(sg.vm0 - vabs) + wave_excl_small(popc3(mm0), &t);
+    const int32_t r0m = (int32_t)(sg.v0m - vabs) + wave_excl_small(popc3(m0m), &t);
     const int first = wave_excl_small(__popc(mask), &total);
 
-    if (mask) {
-        L.c[lane] = RECOMP ? c : sample(in, px, py, pz);
-        const V3<S> f = deform_at<S, DEF>(in, px, py, pz);
-        L.fc[0][lane] = f.x;
-        L.fc[1][lane] = f.y;
-        L.fc[2][lane] = f.z;
-    }
     L.id[0][lane] = r00;
     L.id[1][lane] = r00 + (m00 & 1);
     L.id[2][lane] = r00 + (m00 & 1) + ((m00 >> 1) & 1);
@@ -1372,14 +1479,12 @@ __device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, i
         if (k < total) {
             const int it = L.item[k], src = it >> 3, slot = it & 7;
             const int axis = slot % 3, end = slot >= 3;
+            // the edge runs from (x, y, z) along `axis`; the vertex is its start (slots 0-2) or its end
             const int x = px - (slot == 3), y = py - (slot == 4), z = z0 + src - (slot == 5);
-            // the edge's other endpoint
-            const int ex = end ? x : x + (axis == 0), ey = end ? y : y + (axis == 1), ez = end ? z : z + (axis == 2);
-            const S dv = L.c[src], dox = sample(in, ex, ey, ez);
-            const S d0 = end ? dox : dv, d1 = end ? dv : dox;
-            const V3<S> fv = {L.fc[0][src], L.fc[1][src], L.fc[2][src]};
-            const V3<S> fo = deform_at<S, DEF>(in, ex, ey, ez);
-            L.res[lane] = edge_adjoint<S, DEF>(x, y, z, axis, end, d0, d1, end ? fo : fv, end ? fv : fo, iso,
+            const int ex = x + (axis == 0), ey = y + (axis == 1), ez = z + (axis == 2);
+            const S d0 = sample(in, x, y, z), d1 = sample(in, ex, ey, ez);
+            const V3<S> f0 = deform_at<S, DEF>(in, x, y, z), f1 = deform_at<S, DEF>(in, ex, ey, ez);
+            L.res[lane] = edge_adjoint<S, DEF>(x, y, z, axis, end, d0, d1, f0, f1, iso,
                                                load_adj(adj, vbase + L.id[slot][src]));
         }
         wave_lds_sync();
@@ -1394,6 +1499,29 @@ __device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, i
         wave_lds_sync();
     }
     store_grad(in, adj_sdf, adj_deform, px, py, pz, acc, adj.flags);
+}
+
+// The touched segments of a block (bits of `act`, segment lo + bit) dealt round-robin over its
+// waves, each wave's next segment loaded while it gathers the current one.
+template <typename S, bool DEF, bool RECOMP>
+__device__ __forceinline__ void bwd_run(const In<S> &in, const SegCtx &sc, int64_t lo, const SegPos &lp, uint64_t act,
+                                        const AdjIn<S> &adj, S *__restrict__ adj_sdf, S *__restrict__ adj_deform,
+                                        BwdLds<S> &L, int wave, int lane) {
+    uint64_t mine = 0;
+    for (int k = 0; act; ++k, act &= act - 1)
+        if ((k & (kWavesPerBlock - 1)) == wave) mine |= act & (0ull - act);
+    if (!mine) return;
+    int i = __builtin_ctzll(mine);
+    BwdSegIn nxt = bwd_load<RECOMP>(in.geo, sc, lo + i, pos_of_lane(lp, i), lane);
+    while (mine) {
+        const BwdSegIn cur = nxt;
+        mine &= mine - 1;
+        if (mine) {
+            i = __builtin_ctzll(mine);
+            nxt = bwd_load<RECOMP>(in.geo, sc, lo + i, pos_of_lane(lp, i), lane);
+        }
+        bwd_segment<S, DEF, RECOMP>(in, sc, cur, adj, adj_sdf, adj_deform, L, lane);
+    }
 }
 
 template <typename S, bool DEF, bool RECOMP>
@@ -1417,10 +1545,7 @@ __global__ __launch_bounds__(kBlock) void bwd_mc_kernel(In<S> in, const SegBase 
             store_grad(in, adj_sdf, adj_deform, sp.px, sp.py, sp.chunk * kWave + lane, Grad<S>{S(0), S(0), S(0), S(0)},
                        adj.flags);
         }
-    for (int k = 0; act; ++k, act &= act - 1)
-        if ((k & (kWavesPerBlock - 1)) == wave)
-            bwd_segment<S, DEF, RECOMP>(in, sc, lo + __builtin_ctzll(act), pos_of_lane(lp, __builtin_ctzll(act)), adj,
-                                        adj_sdf, adj_deform, s_lds[wave], lane);
+    bwd_run<S, DEF, RECOMP>(in, sc, lo, lp, act, adj, adj_sdf, adj_deform, s_lds[wave], wave, lane);
 }
 
 // Per-edge upstream gradient of a DMC forward: ge[q] = sum over the quad's corners v of
@@ -1449,13 +1574,16 @@ __device__ __forceinline__ void edge_grad_one(const void *__restrict__ quads, in
 
 template <typename S>
 __global__ __launch_bounds__(kBlock) void dmc_edge_grad_kernel(const void *__restrict__ quads, int64_t nq,
+                                                               const SegBase *__restrict__ tot,
                                                                const uint8_t *__restrict__ vinfo, AdjIn<S> adj,
                                                                S *__restrict__ ge) {
     __shared__ S s_rcp[8];
     if (threadIdx.x < 8) s_rcp[threadIdx.x] = threadIdx.x ? S(1) / S(threadIdx.x) : S(0);
     __syncthreads();
+    // nq may be a capacity: the quads are the first tot->v rows (the count pass's total)
+    const int64_t n = tot->v < nq ? tot->v : nq;
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (q < nq) edge_grad_one(quads, q, vinfo, adj, s_rcp, ge);
+    if (q < n) edge_grad_one(quads, q, vinfo, adj, s_rcp, ge);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1539,12 +1667,23 @@ __global__ __launch_bounds__(kBlock) void emit_mc_batch_kernel(const BatchGrid<S
     const int64_t lo = (b - rf[j]) * run, hi = lo + run < in.geo.nseg ? lo + run : in.geo.nseg;
     uint64_t act = run_active(sc.seg, lo, hi, lane);
     const SegPos lp = seg_pos(in.geo, lo + lane);
+    // this wave's segments: every kWavesPerBlock-th active one; loads run one segment ahead
+    uint64_t mine = 0;
     for (int k = 0; act; ++k, act &= act - 1)
-        if ((k & (kWavesPerBlock - 1)) == wave) {
-            const int i = __builtin_ctzll(act);
-            emit_mc_segment<S, DEF>(in, sc, lo + i, pos_of_lane(lp, i), G.verts, G.faces, out, G.nm, s_case,
-                                    s_lds[wave], lane);
+        if ((k & (kWavesPerBlock - 1)) == wave) mine |= act & (0ull - act);
+    if (mine) {
+        int i = __builtin_ctzll(mine);
+        McSegIn nxt = mc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
+        while (mine) {
+            const McSegIn cur = nxt;
+            mine &= mine - 1;
+            if (mine) {
+                i = __builtin_ctzll(mine);
+                nxt = mc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
+            }
+            emit_mc_segment<S, DEF>(in, sc, cur, G.verts, G.faces, out, G.nm, s_case, s_lds[wave], lane);
         }
+    }
 }
 
 template <typename S, bool DEF>
@@ -1553,11 +1692,8 @@ __global__ __launch_bounds__(kBlock) void emit_dmc_batch_kernel(const BatchGrid<
                                                                 const int64_t *__restrict__ rf, int m,
                                                                 const SegBase *__restrict__ seg,
                                                                 const uint16_t *__restrict__ info, int run, int64_t nblocks) {
-    __shared__ DmcTables<S> T;
     __shared__ DmcEmitLds<S> s_lds[kWavesPerBlock];
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
-    load_dmc_tables(T, tid);
-    __syncthreads();
     const int64_t b = xcd_block(blockIdx.x, nblocks);
     const int j = find_group(rf, m, b);
     const BatchGrid<S> &G = grids[gl[j]];
@@ -1568,12 +1704,22 @@ __global__ __launch_bounds__(kBlock) void emit_dmc_batch_kernel(const BatchGrid<
     const int64_t lo = (b - rf[j]) * run, hi = lo + run < in.geo.nseg ? lo + run : in.geo.nseg;
     uint64_t act = run_active(sc.seg, lo, hi, lane);
     const SegPos lp = seg_pos(in.geo, lo + lane);
+    uint64_t mine = 0;
     for (int k = 0; act; ++k, act &= act - 1)
-        if ((k & (kWavesPerBlock - 1)) == wave) {
-            const int i = __builtin_ctzll(act);
-            emit_dmc_segment<S, DEF>(in, sc, lo + i, pos_of_lane(lp, i), (S *)G.verts, G.faces, G.vinfo, out, G.nm,
-                                     T, s_lds[wave], lane);
+        if ((k & (kWavesPerBlock - 1)) == wave) mine |= act & (0ull - act);
+    if (mine) {
+        int i = __builtin_ctzll(mine);
+        DmcSegIn nxt = dmc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
+        while (mine) {
+            const DmcSegIn cur = nxt;
+            mine &= mine - 1;
+            if (mine) {
+                i = __builtin_ctzll(mine);
+                nxt = dmc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
+            }
+            emit_dmc_segment<S, DEF>(in, sc, cur, (S *)G.verts, G.faces, G.vinfo, out, G.nm, s_lds[wave], lane);
         }
+    }
 }
 
 // Per-edge upstream gradients of every DMC grid of the batch: blocks of kBlock quads of one grid.
@@ -1624,10 +1770,7 @@ __global__ __launch_bounds__(kBlock) void bwd_batch_kernel(const BatchGrid<S> *_
             store_grad(in, G.adj_sdf, G.adj_deform, sp.px, sp.py, sp.chunk * kWave + lane,
                        Grad<S>{S(0), S(0), S(0), S(0)}, adj.flags);
         }
-    for (int k = 0; act; ++k, act &= act - 1)
-        if ((k & (kWavesPerBlock - 1)) == wave)
-            bwd_segment<S, DEF, RECOMP>(in, sc, lo + __builtin_ctzll(act), pos_of_lane(lp, __builtin_ctzll(act)), adj,
-                                        G.adj_sdf, G.adj_deform, s_lds[wave], lane);
+    bwd_run<S, DEF, RECOMP>(in, sc, lo, lp, act, adj, G.adj_sdf, G.adj_deform, s_lds[wave], wave, lane);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1935,7 +2078,7 @@ int launch_backward(int algo, const void *sdf, const void *deform, double iso, i
         if (nfaces > 0) {
             const int64_t eb = (nfaces + kBlock - 1) / kBlock;
             hipLaunchKernelGGL((dmc_edge_grad_kernel<S>), dim3((unsigned)eb), dim3(kBlock), 0, st, faces, nfaces,
-                               (const uint8_t *)vinfo, a, (S *)scratch);
+                               sb + g.nseg, (const uint8_t *)vinfo, a, (S *)scratch);
             int rc = check_hip(hipGetLastError(), "edge gradient kernel launch");
             if (rc) return rc;
         }
@@ -2278,6 +2421,12 @@ int64_t diso_seg_bytes(int64_t X, int64_t Y, int64_t Z, int flags) {
     Geo g;
     if (!make_geo(X, Y, Z, flags, &g)) return fail(DISO_E_INVALID, "unsupported grid dims");
     return (flags & DISO_FLAG_RECOMPUTE) ? bases_bytes_for(g) : seg_bytes_for(g);
+}
+
+int64_t diso_totals_offset(int64_t X, int64_t Y, int64_t Z, int flags) {
+    Geo g;
+    if (!make_geo(X, Y, Z, flags, &g)) return fail(DISO_E_INVALID, "unsupported grid dims");
+    return g.nseg * (int64_t)sizeof(SegBase);
 }
 
 int64_t diso_ctl_bytes(int64_t X, int64_t Y, int64_t Z, int flags) {

@@ -84,6 +84,11 @@ typedef struct {
  * is that of the bases prefix alone, which is all a DISO_FLAG_RECOMPUTE backward reads. */
 int64_t diso_seg_bytes(int64_t X, int64_t Y, int64_t Z, int flags);
 
+/* Byte offset in `seg` of the totals a diso_count leaves on the device: four int64, (crossing
+ * edges, MC triangles | DMC vertices, DISO_TOTAL_* flags, algo). The reference keeps these counts
+ * in host members of CuMC / CUDualMC (src/cumc.h:73-99); here they can stay on the device. */
+int64_t diso_totals_offset(int64_t X, int64_t Y, int64_t Z, int flags);
+
 /* Size in bytes of the control workspace ("ctl") that serves a grid of dims (X,Y,Z) or any
  * smaller one. It must be zero-filled once when allocated and is then reused across calls; calls
  * that share a ctl must not overlap in time (it holds the single-pass scan's tile states). */
@@ -94,7 +99,10 @@ int64_t diso_ctl_bytes(int64_t X, int64_t Y, int64_t Z, int flags);
  * count_cell_mc_verts_kernel, count_cell_mc_tris_kernel / count_cell_patches_kernel and the three
  * CUB scans of CuMC::forward (src/cumc.cu:651-723) and CUDualMC::forward
  * (src/cudualmc.cu:1058-1121). If totals_host is not NULL the totals are copied back and the
- * stream is synchronised (the one host sync the plain API needs to size its outputs). */
+ * stream is synchronised (the one host sync the plain API needs to size its outputs). With
+ * totals_host NULL the call only enqueues work: the totals stay in `seg` (diso_totals_offset), and
+ * diso_emit / diso_backward take capacities instead of counts, so a whole step can be captured in
+ * a HIP graph. */
 int diso_count(int algo, int dtype, const void *sdf, int64_t X, int64_t Y, int64_t Z, double iso,
                int flags, void *seg, int64_t seg_bytes, void *ctl, int64_t ctl_bytes,
                diso_totals_t *totals_host, void *stream);
@@ -126,14 +134,14 @@ int diso_emit(int algo, int dtype, const void *sdf, const void *deform, int64_t 
  * crossing edge touches it) unless DISO_FLAG_ACCUMULATE is set. `seg` must hold the result of a
  * diso_count on the same inputs. DMC only: `faces` / `vinfo` are the quads (in the index width
  * of `flags`) and vertex info written by diso_emit with the same inputs, n_faces the quad count,
- * and `scratch` (diso_scratch_bytes) is overwritten. */
+ * and `scratch` (diso_scratch_bytes) is overwritten. n_faces and n_verts may be capacities (rows
+ * allocated) rather than the counts: rows past the counts in `seg` are never read. */
 int diso_backward(int algo, int dtype, const void *sdf, const void *deform, int64_t X, int64_t Y,
                   int64_t Z, double iso, int flags, const void *seg, int64_t seg_bytes,
                   const void *faces, int64_t n_faces, const void *vinfo, const void *adj_verts,
                   int64_t n_verts, void *adj_sdf, void *adj_deform, void *scratch,
                   int64_t scratch_bytes, void *stream);
 
-/* Reads the totals of the last diso_count on `seg` (blocking copy on `stream`). */
 /* Size in bytes of the scratch buffer diso_split needs for n_quads quads. */
 int64_t diso_split_scratch_bytes(int64_t n_quads);
 
@@ -193,6 +201,7 @@ enum { DISO_PARAM_RUN = 0, DISO_PARAM_COUNT_BLOCKS_PER_CU = 1 };
 int diso_set_param(int which, int value);
 int diso_get_param(int which);
 
+/* Reads the totals of the last diso_count on `seg` (blocking copy on `stream`). */
 int diso_read_totals(const void *seg, int64_t seg_bytes, int64_t X, int64_t Y, int64_t Z,
                      int flags, diso_totals_t *totals_host, void *stream);
 

@@ -40,6 +40,9 @@ def test_workspace_sizes():
     assert seg == (nseg + 2) * 16 + nseg * 64 * 2
     # a recompute-classification backward keeps only the bases
     assert L.diso_seg_bytes(n, n, n, _lib.FLAG_VIRTUAL_PAD | _lib.FLAG_RECOMPUTE) == (nseg + 2) * 16
+    # the totals follow the bases, inside the recompute prefix
+    assert L.diso_totals_offset(n, n, n, _lib.FLAG_VIRTUAL_PAD) == nseg * 16
+    assert L.diso_totals_offset(0, 4, 4, 0) < 0
     nseg = n * n * ((n + 63) // 64)
     assert L.diso_seg_bytes(n, n, n, 0) == (nseg + 2) * 16 + nseg * 64 * 2
     assert L.diso_ctl_bytes(n, n, n, 0) > 0
