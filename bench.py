@@ -20,6 +20,7 @@ import ctypes
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -43,6 +44,12 @@ def parse():
     p.add_argument("--copy-mib", type=int, default=256)
     p.add_argument("--batch-count", type=int, default=64, help="grids of the uneven batch (0: skip it)")
     p.add_argument("--batch-reps", type=int, default=5)
+    p.add_argument("--learn-n", type=int, default=512, help="learnable-sdf fit grid (0: skip the fit)")
+    p.add_argument("--learn-stored-n", type=int, default=384)
+    p.add_argument("--learn-steps", type=int, default=100)
+    p.add_argument("--serve-requests", type=int, default=20000, help="requests of the serving stream (0: skip it)")
+    p.add_argument("--serve-load", type=float, default=0.95, help="arrival rate / the plain path's service rate")
+    p.add_argument("--serve-workers", type=int, default=2)
     return p.parse_args()
 
 
@@ -226,6 +233,144 @@ def batched_vs_plain(dev, count, seed, reps):
     }
 
 
+def learnable_fit(dev, n, n_stored, steps):
+    """The differentiation-at-scale workload (diso_amd.fit): a learnable fp32 sdf (sphere r=0.4 +
+    N(0, 0.01)) and deform (U(-0.1, 0.1)) fitted by Adam (lr 1e-3) to a sphere of radius 0.3
+    through DiffDMC vertices. At n^3 with the recompute backward and at n_stored^3 with the stored
+    classification: a few eager steps, then `steps` replays of the graph-captured step, timed with
+    HIP events on the replay stream."""
+    import torch
+
+    from diso_amd.fit import SdfFit
+
+    out = {}
+    for label, size, recompute in (("recompute", n, True), ("stored", n_stored, False)):
+        torch.cuda.reset_peak_memory_stats(dev)
+        fit = SdfFit(size, device=dev, seed=0, recompute=recompute)
+        fit.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            fit.step()
+        torch.cuda.synchronize()
+        eager_ms = (time.perf_counter() - t0) / 3 * 1e3
+        fit.capture(warmup=3)
+        torch.cuda.synchronize()
+        loss0 = float(fit.loss)
+        st = torch.cuda.current_stream(dev)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(steps):
+            fit.replay()
+        e1.record(st)
+        torch.cuda.synchronize()
+        fit.check()
+        ms = e0.elapsed_time(e1) / steps
+        out[label] = {"n": size, "recompute_backward": recompute, "graph_ms_per_step": round(ms, 3),
+                      "eager_ms_per_step": round(eager_ms, 3), "gvoxel_s": round(size ** 3 / (ms / 1e3) / 1e9, 3),
+                      "loss_before": round(loss0, 6), "loss_after": round(float(fit.loss), 6),
+                      "dmc_verts": int(fit.counts[0]), "vert_capacity": fit.ex.vert_capacity,
+                      "peak_alloc_gib": round(torch.cuda.max_memory_allocated(dev) / 2 ** 30, 2)}
+        del fit
+        torch.cuda.empty_cache()
+    out["workload"] = (f"learnable sdf sphere r=0.4 + N(0,0.01), deform U(-0.1,0.1), Adam lr 1e-3, "
+                       f"loss mean((|v-0.5|-0.3)^2) over DiffDMC verts (normalize=True), {steps} graph-captured "
+                       f"steps per grid after 7 eager ones")
+    return out
+
+
+def serving(dev, count, load, seed, workers, budget_gib=200):
+    """The serving workload (diso_amd.workloads.request_stream): `count` requests, N log-uniform
+    over 32..384, mixed families, MC/DMC 1:1, 30 % with a backward. The plain path serves them one
+    call at a time in arrival order; the server (diso_amd.serve, continuous batching under a
+    `budget_gib` memory budget) takes them as they arrive. Capacity: every request at once (the
+    plain path back to back). Latency (result ready - arrival) under Poisson arrivals at `load`
+    times the plain path's capacity."""
+    import torch
+
+    import diso_amd
+    from diso_amd import workloads
+    from diso_amd.serve import Request, Server
+
+    grids, reqs = workloads.request_stream(count, 1.0, seed=seed, device=dev)  # arrivals rescaled below
+    mods = {"mc": diso_amd.DiffMC(torch.float32), "dmc": diso_amd.DiffDMC(torch.float32)}
+    st = torch.cuda.current_stream(dev)
+    voxels = sum(grids[i][3] ** 3 for _, i, _, _ in reqs)
+
+    def plain(i, algo, bwd):
+        sdf, deform, _, _ = grids[i]
+        s, d = sdf.detach().requires_grad_(bwd), deform.detach().requires_grad_(bwd)
+        v, f = mods[algo](s, d, return_quads=True) if algo == "dmc" else mods[algo](s, d)
+        if bwd and v.shape[0]:
+            torch.autograd.grad([v], [s, d], grad_outputs=[torch.randn_like(v)])
+        st.synchronize()
+
+    def wait_until(t0, t):
+        dt = t - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+
+    def run_plain(arrivals):
+        lat = []
+        t0 = time.perf_counter()
+        for t, (_, i, a, b) in zip(arrivals, reqs):
+            wait_until(t0, t)
+            plain(i, a, b)
+            lat.append(time.perf_counter() - t0 - t)
+        return time.perf_counter() - t0, np.array(lat) * 1e3
+
+    def run_served(arrivals):
+        # results are dropped as they complete (a client consumes them): only in-flight work holds memory
+        finished = threading.Event()
+        state = {"left": count, "error": None}
+        lock = threading.Lock()
+
+        def on_done(fut):
+            with lock:
+                if fut.exception() is not None and state["error"] is None:
+                    state["error"] = fut.exception()
+                state["left"] -= 1
+                if state["left"] == 0:
+                    finished.set()
+
+        with Server(budget_bytes=budget_gib << 30, workers=workers, device=dev) as srv:
+            t0 = time.perf_counter()
+            for t, (_, i, a, b) in zip(arrivals, reqs):
+                wait_until(t0, t)
+                sdf, deform, _, _ = grids[i]
+                up = (lambda nv: torch.randn((nv, 3), device=dev)) if b else None
+                srv.submit(Request(sdf, a, deform, upstream=up)).add_done_callback(on_done)
+            finished.wait()
+            span = time.perf_counter() - t0
+            if state["error"] is not None:
+                raise state["error"]
+            return span, np.array(srv.latencies) * 1e3, srv.batches
+
+    for _, i, a, b in reqs[:20]:
+        plain(i, a, b)
+    now = [0.0] * count
+    plain_cap, _ = run_plain(now)
+    served_cap, _, _ = run_served(now)
+    rate = load * count / plain_cap
+    arrivals = [t / rate for t, *_ in reqs]  # unit-rate Poisson times, rescaled
+    plain_span, plain_lat = run_plain(arrivals)
+    served_span, served_lat, nb = run_served(arrivals)
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3)
+    return {
+        "workload": f"{count} requests, N log-uniform 32..384 (pool of 48 grids), sphere / rounded box / blobs / noise, "
+                    "deform U(-0.3,0.3), MC/DMC 1:1, 30% with backward, 200 GiB budget; capacity with every request "
+                    f"queued at once, latency under Poisson arrivals at {rate:.0f}/s ({load} x the plain capacity)",
+        "voxels": int(voxels),
+        "rate_per_s": round(rate, 1),
+        "capacity_gvoxel_s": {"served": round(voxels / served_cap / 1e9, 3), "plain": round(voxels / plain_cap / 1e9, 3),
+                              "speedup": round(plain_cap / served_cap, 3)},
+        "served": {"p50_ms": pct(served_lat, 50), "p99_ms": pct(served_lat, 99), "batches": nb,
+                   "gvoxel_s": round(voxels / served_span / 1e9, 3), "workers": workers},
+        "plain": {"p50_ms": pct(plain_lat, 50), "p99_ms": pct(plain_lat, 99),
+                  "gvoxel_s": round(voxels / plain_span / 1e9, 3)},
+    }
+
+
 def cpu_baseline(sdf_np, deform_np, n, seed, reps):
     from diso_amd import workloads
     from oracle import oracle as O
@@ -352,6 +497,12 @@ def main():
         del sets, mods
         torch.cuda.empty_cache()
         out["batched"] = batched_vs_plain(dev, args.batch_count, args.seed, args.batch_reps)
+    if args.serve_requests > 0:
+        torch.cuda.empty_cache()
+        out["serving"] = serving(dev, args.serve_requests, args.serve_load, args.seed, args.serve_workers)
+    if args.learn_n > 0:
+        torch.cuda.empty_cache()
+        out["learn"] = learnable_fit(dev, args.learn_n, args.learn_stored_n, args.learn_steps)
     if not args.no_cpu_baseline:
         sdf_np, def_np, _ = workloads.noisy_round_box(n, seed=args.seed)
         out["cpu_baseline"] = cpu_baseline(sdf_np, def_np, n, args.seed, args.cpu_reps)

@@ -58,12 +58,15 @@ class _BatchExtract(torch.autograd.Function):
         dev = sdfs[0].device
         stream = _stream(dev)
         d = st.desc
-        nbytes = L.diso_batch_seg_bytes(ctypes.addressof(d), n)
-        _lib.check(0 if nbytes >= 0 else nbytes, "batch workspace sizing")
-        st.seg = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-        ctl = st.ex.ctl_for(dev, n)
-        rc = L.diso_batch_count(code, ctypes.addressof(d), n, _ptr(st.seg), nbytes, _ptr(ctl), ctl.numel(), stream)
-        _lib.check(rc, "batch count pass")
+        if st.seg is None:  # else counted already (BatchExtractor.count)
+            nbytes = L.diso_batch_seg_bytes(ctypes.addressof(d), n)
+            _lib.check(0 if nbytes >= 0 else nbytes, "batch workspace sizing")
+            st.seg = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+            ctl = st.ex.ctl_for(dev, n)
+            rc = L.diso_batch_count(code, ctypes.addressof(d), n, _ptr(st.seg), nbytes, _ptr(ctl), ctl.numel(),
+                                    stream)
+            _lib.check(rc, "batch count pass")
+        nbytes = st.seg.numel()
         both = _lib.TOTAL_HAS_BELOW | _lib.TOTAL_HAS_ABOVE
         verts, faces = [], []
         for i in range(n):
@@ -132,6 +135,15 @@ class _BatchExtract(torch.autograd.Function):
         return (None, *g_sdf, *g_def)
 
 
+class _Counted:
+    """Handle of a counting pass (BatchExtractor.count)."""
+
+    def __init__(self, st, sizes, key):
+        self.st = st
+        self.sizes = sizes
+        self.key = key
+
+
 class BatchExtractor:
     """Extracts a list of grids in one batch. `algos` holds "mc" / "dmc" per grid; `deforms` is
     None or a list with a [X,Y,Z,3] tensor or None per grid. All grids share the dtype, the
@@ -157,10 +169,8 @@ class BatchExtractor:
             self._ctl[device.index] = ctl
         return ctl
 
-    def __call__(self, grids, algos, deforms=None, isovalue=0.0, normalize=True, return_quads=False):
+    def _describe(self, grids, algos, deforms, isovalue, normalize):
         n = len(grids)
-        if n == 0:
-            return []
         if len(algos) != n:
             raise ValueError("one algorithm per grid")
         deforms = [None] * n if deforms is None else list(deforms)
@@ -200,6 +210,47 @@ class BatchExtractor:
             dsc.iso = float(isovalue)
             dsc.sdf = g.data_ptr()
             dsc.deform = None if df is None else df.data_ptr()
+        return st, sdfs, defs, dev
+
+    def count(self, grids, algos, isovalue=0.0, normalize=True):
+        """The counting pass alone, before any output is allocated: returns a handle whose
+        `sizes` holds per grid (n_verts, n_faces, flags) as the extraction will produce them
+        (DMC faces are quads). Passing the handle as `counted` to a call on the same grids,
+        algorithms, isovalue and normalisation reuses this pass. Costs the workspace and one host
+        synchronisation."""
+        if not grids:
+            return _Counted(None, [], [])
+        st, sdfs, _, dev = self._describe(grids, algos, None, isovalue, normalize)
+        L = _lib.lib()
+        d = st.desc
+        with torch.cuda.device(dev):
+            nbytes = L.diso_batch_seg_bytes(ctypes.addressof(d), st.n)
+            _lib.check(0 if nbytes >= 0 else nbytes, "batch workspace sizing")
+            st.seg = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+            ctl = self.ctl_for(dev, st.n)
+            rc = L.diso_batch_count(self.code, ctypes.addressof(d), st.n, _ptr(st.seg), nbytes, _ptr(ctl),
+                                    ctl.numel(), _stream(dev))
+            _lib.check(rc, "batch count pass")
+        sizes = [(int(d[i].totals.n_verts), int(d[i].totals.n_faces), int(d[i].totals.flags)) for i in range(st.n)]
+        return _Counted(st, sizes, [(g.data_ptr(), int(d[i].algo), tuple(g.shape)) for i, g in enumerate(sdfs)])
+
+    def __call__(self, grids, algos, deforms=None, isovalue=0.0, normalize=True, return_quads=False, counted=None):
+        n = len(grids)
+        if n == 0:
+            return []
+        st, sdfs, defs, dev = self._describe(grids, algos, deforms, isovalue, normalize)
+        if counted is not None:
+            # reuse the counting pass: same grids (storage, dims, algorithm) and settings
+            same = counted.st is not None and counted.st.n == n and all(
+                (g.data_ptr(), int(st.desc[i].algo), tuple(g.shape)) == counted.key[i] and
+                counted.st.desc[i].flags == st.desc[i].flags and counted.st.desc[i].iso == st.desc[i].iso
+                for i, g in enumerate(sdfs))
+            if not same:
+                raise ValueError("counted: the handle's counting pass was not run on these grids and settings")
+            for i in range(n):
+                counted.st.desc[i].deform = st.desc[i].deform
+            st = counted.st
+            counted.st = None  # a handle serves one call
         with torch.cuda.device(dev):
             outs = _BatchExtract.apply(st, *sdfs, *defs)
             verts, faces = list(outs[:n]), list(outs[n:])

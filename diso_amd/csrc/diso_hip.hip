@@ -174,14 +174,17 @@ __device__ __forceinline__ int dmc_code(const Corners<S> &k, S iso) {
 template <typename S>
 __device__ __forceinline__ int dmc_good_code(const In<S> &in, const uint64_t *cases, int code, int x,
                                              int y, int z) {
-    int dir = DISO_DMC_DIR(cases[code]);
+    const int dir = DISO_DMC_DIR(cases[code]);
     if (dir != 7) {
-        int n[3] = {x, y, z};
-        int comp = dir >> 1;
-        n[comp] += (dir & 1) ? 1 : -1;
-        int lim = comp == 0 ? in.geo.g0 : (comp == 1 ? in.geo.g1 : in.geo.g2);
-        if (n[comp] >= 0 && n[comp] < lim - 1) {
-            int ncode = dmc_code(load_corners(in, n[0], n[1], n[2]), in.iso);
+        // neighbour along axis comp, +1 or -1; selected arithmetically (no indexed arrays, so the
+        // grid parameters can stay in registers)
+        const int comp = dir >> 1, step = (dir & 1) ? 1 : -1;
+        const int on0 = comp == 0, on1 = comp == 1, on2 = comp == 2;
+        const int nx = x + on0 * step, ny = y + on1 * step, nz = z + on2 * step;
+        const int c = on0 * nx + on1 * ny + on2 * nz;
+        const int lim = on0 * in.geo.g0 + on1 * in.geo.g1 + on2 * in.geo.g2;
+        if (c >= 0 && c < lim - 1) {
+            const int ncode = dmc_code(load_corners(in, nx, ny, nz), in.iso);
             if (DISO_DMC_DIR(cases[ncode]) != 7) code ^= 0xff;
         }
     }
@@ -515,7 +518,7 @@ struct CountSrc {
 };
 
 template <typename S, int ALGO, bool BATCH>
-__global__ __launch_bounds__(kCountBlock, kCountBlocksPerCU * kCountWaves / 4) void count_kernel(
+__global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCountWaves / 4) void count_kernel(
     CountSrc<S> src, const BatchGrid<S> *__restrict__ grids, SegBase *seg, uint16_t *info, uint8_t *ctl,
     int64_t nblocks) {
     __shared__ uint64_t s_case[BATCH ? 2 : 1][256];
@@ -553,56 +556,53 @@ __global__ __launch_bounds__(kCountBlock, kCountBlocksPerCU * kCountWaves / 4) v
     int64_t wv = 0, wf = 0;
     uint32_t wfl = 0;
     const int64_t wlo = lo + (hi - lo) * wave / kCountWaves, whi = lo + (hi - lo) * (wave + 1) / kCountWaves;
-    int gi = 0;        // BATCH: grid of the current segment (segments only increase)
-    uint32_t gfl = 0;  // BATCH: its sample flags so far
-    In<S> in = src.in;
-    if constexpr (BATCH) {
+    // BATCH: the wave's range may span several grids; one pass of the outer loop per grid, with
+    // that grid's parameters held in scalar registers for the inner loop.
+    int gi = 0;
+    if constexpr (BATCH)
         while (gi + 1 < src.ngrids && wlo >= grids[gi + 1].seg_first) ++gi;
-        in = load_uniform(&grids[gi].in);
-    }
-    SegPos sp = seg_pos(in.geo, wlo - (BATCH ? grids[gi].seg_first : 0));
-    for (int64_t s = wlo; s < whi; ++s) {  // wave-uniform
-        int nv, nf;
-        uint32_t fl, iw;
-        if constexpr (BATCH) {
-            if (gi + 1 < src.ngrids && s == grids[gi + 1].seg_first) {  // next grid
-                gfl = wave_or(gfl);
-                if (lane == 0 && gfl) atomicOr(src.gflags + gi, gfl);
-                gfl = 0;
-                ++gi;
-                in = load_uniform(&grids[gi].in);
-                sp = SegPos{0, 0, 0};
+    for (int64_t s = wlo; s < whi;) {  // wave-uniform
+        In<S> in;
+        if constexpr (BATCH)
+            in = load_uniform(&grids[gi].in);
+        else
+            in = src.in;
+        const int algo = BATCH ? grids[gi].algo : ALGO;
+        const int64_t first = BATCH ? grids[gi].seg_first : 0;
+        const int64_t end = (BATCH && gi + 1 < src.ngrids && grids[gi + 1].seg_first < whi) ? grids[gi + 1].seg_first
+                                                                                          : whi;
+        const uint64_t *cases = s_case[(BATCH && algo != DISO_MC) ? 1 : 0];
+        uint32_t gfl = 0;  // the grid's sample flags
+        SegPos sp = seg_pos(in.geo, s - first);
+        for (; s < end; ++s) {
+            int nv, nf;
+            uint32_t fl, iw;
+            CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
+            w.template load_box<0, 2, 0, 2, 0, 2>(in);
+            count_position<S>(in, algo, cases, w, &nv, &nf, &fl, &iw);
+            if (++sp.chunk == in.geo.nchunk) {  // next segment of the grid
+                sp.chunk = 0;
+                if (++sp.py == in.geo.g1) {
+                    sp.py = 0;
+                    ++sp.px;
+                }
             }
-        }
-        CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
-        w.template load_box<0, 2, 0, 2, 0, 2>(in);
-        if constexpr (BATCH) {
-            const int algo = grids[gi].algo;
-            count_position<S>(in, algo, s_case[algo == DISO_MC ? 0 : (BATCH ? 1 : 0)], w, &nv, &nf, &fl, &iw);
+            info[s * kWave + lane] = (uint16_t)iw;
+            const int tv = wave_total_small(nv), tf = wave_total_small(nf);
+            if (lane == 0) {
+                seg[s].v = tv;
+                seg[s].f = tf;
+            }
+            wv += tv;
+            wf += tf;
             gfl |= fl;
-        } else {
-            count_position<S>(in, ALGO, s_case[0], w, &nv, &nf, &fl, &iw);
         }
-        if (++sp.chunk == in.geo.nchunk) {  // next segment of the grid
-            sp.chunk = 0;
-            if (++sp.py == in.geo.g1) {
-                sp.py = 0;
-                ++sp.px;
-            }
+        wfl |= gfl;
+        if constexpr (BATCH) {
+            gfl = wave_or(gfl);
+            if (lane == 0 && gfl) atomicOr(src.gflags + gi, gfl);
+            ++gi;
         }
-        info[s * kWave + lane] = (uint16_t)iw;
-        const int tv = wave_total_small(nv), tf = wave_total_small(nf);
-        if (lane == 0) {
-            seg[s].v = tv;
-            seg[s].f = tf;
-        }
-        wv += tv;
-        wf += tf;
-        wfl |= fl;
-    }
-    if constexpr (BATCH) {
-        gfl = wave_or(gfl);
-        if (lane == 0 && gfl) atomicOr(src.gflags + gi, gfl);
     }
     wfl = wave_or(wfl);
     if (lane == 0) {

@@ -2,12 +2,16 @@
 
 Requests (a grid already on the device, MC or DMC, optionally a backward with an upstream
 gradient) are queued by any thread. Each worker owns a HIP stream and a BatchExtractor; when free,
-it takes every admissible queued request (up to `max_batch`) and runs them as one batch, so work
-that arrives while other batches are on the device is admitted as soon as a worker and memory free
-up, and the batches of different workers overlap on the device. Admission is by a device-memory
-budget: a request is charged its workspace, its outputs and gradients, estimated before the count
-pass and corrected from the count pass's exact totals; a request larger than the whole budget runs
-alone. Results (and latencies, submit -> result ready) come back through futures.
+it takes the queued requests that share the oldest one's settings (up to `max_batch`) and runs
+them as one batch, so work that arrives while other batches are on the device is admitted as soon
+as a worker and memory free up, and the batches of different workers overlap on the device.
+
+Admission is by a device-memory budget, in two steps. Taking a request charges its counting
+workspace (known from its dims). The batch's counting pass then gives every request's exact
+vertex and face counts, hence its exact footprint (workspace, outputs, and for a backward the
+upstream gradient, scratch and input gradients); the requests that fit the free budget, in queue
+order, run, and the others go back to the head of the queue. A request larger than the whole
+budget runs alone. Results (and latencies, submit -> result ready) come back through futures.
 
 Each result equals what DiffMC / DiffDMC return for that request alone, bit for bit: a batch runs
 the same kernels per grid (diso_amd/batch.py).
@@ -43,28 +47,40 @@ class Request:
         """Requests batched together share these (one BatchExtractor call)."""
         return (self.isovalue, self.normalize, self.return_quads)
 
-    def sizes(self):
-        """Device bytes the engine holds for this request: (workspace, outputs, gradients), the
-        outputs sized for a dense surface (at most 12 B per padded position for vertices and
-        faces) until the count pass gives the exact ones."""
+    def workspace(self):
+        """Device bytes of the counting pass's workspace: a 16-bit word per padded position and a
+        16-byte base per 64 positions."""
         X, Y, Z = (int(d) for d in self.sdf.shape)
-        s = self.sdf.element_size()
         pos = (X + 2) * (Y + 2) * (Z + 2)
-        grads = (self.sdf.numel() + (self.deform.numel() if self.deform is not None else 0)) * s \
-            if self.upstream is not None else 0
-        return 3 * pos, 12 * pos * s // 4, grads
+        return 2 * pos + pos // 4 + 4096
 
-    def estimate(self):
-        return sum(self.sizes())
+    def footprint(self, n_verts, n_faces):
+        """Device bytes the engine holds for this request given its counts: workspace, vertices,
+        faces (DMC quads, plus triangles and split scratch without `return_quads`), DMC vertex
+        info, and with a backward the upstream gradient, the DMC edge-gradient scratch and the
+        input gradients."""
+        s = self.sdf.element_size()
+        dmc = self.algo == "dmc"
+        out = 3 * s * n_verts + (32 if dmc else 24) * n_faces + (n_verts if dmc else 0)
+        if dmc and not self.return_quads:
+            out += 49 * n_faces + 16384
+        grads = 0
+        if self.upstream is not None:
+            grads = self.sdf.numel() * s + (self.deform.numel() * s if self.deform is not None else 0)
+            grads += 3 * s * n_verts + (3 * s * n_faces if dmc else 0)
+        return self.workspace() + out + grads
 
 
 class Server:
     def __init__(self, dtype=torch.float32, device="cuda:0", budget_bytes=200 << 30, max_batch=64, workers=2,
-                 reference_compat=False):
+                 reference_compat=False, max_batch_voxels=1 << 22):
         self.dtype = dtype
         self.device = torch.device(device)
         self.budget = int(budget_bytes)
         self.max_batch = int(max_batch)
+        # A batch runs pass by pass over all its grids, so its inputs should stay cache-resident
+        # between the passes: grids are batched up to this many voxels (a larger one runs alone).
+        self.max_batch_voxels = int(max_batch_voxels)
         self.reference_compat = reference_compat
         self._queue = collections.deque()
         self._cv = threading.Condition()
@@ -106,33 +122,72 @@ class Server:
     # ---- worker side
 
     def _admit(self):
-        """Pops the requests the next batch runs: the oldest request always (alone if it exceeds
-        the budget), then every queued request with the same key that fits the remaining budget."""
+        """Pops the requests the next batch counts: the oldest request (if its workspace fits the
+        free budget, or nothing else holds memory), then queued requests with the same settings
+        whose workspace fits, up to `max_batch` requests and `max_batch_voxels` voxels. Charges
+        their workspaces."""
         first = self._queue[0]
         free = self.budget - self._reserved
-        if first.estimate() > free and self._reserved > 0:
+        if first.workspace() > free and self._reserved > 0:
             return []  # wait for memory
         take = [self._queue.popleft()]
-        used = first.estimate()
+        used = first.workspace()
+        voxels = first.sdf.numel()
         keep = collections.deque()
-        while self._queue and len(take) < self.max_batch:
+        while self._queue and len(take) < self.max_batch and voxels < self.max_batch_voxels:
             r = self._queue.popleft()
-            if r.key() == first.key() and used + r.estimate() <= max(free, first.estimate()):
+            if (r.key() == first.key() and used + r.workspace() <= free
+                    and voxels + r.sdf.numel() <= self.max_batch_voxels):
                 take.append(r)
-                used += r.estimate()
+                used += r.workspace()
+                voxels += r.sdf.numel()
             else:
                 keep.append(r)
         keep.extend(self._queue)
         self._queue = keep
         for r in take:
-            r.charge = r.estimate()
+            r.charge = r.workspace()
         self._reserved += used
         self.peak_reserved = max(self.peak_reserved, self._reserved)
         return take
 
+    def _fit(self, batch, sizes):
+        """With the exact counts: charges the footprints of the longest queue-order prefix of the
+        batch that fits the free budget and returns the rest to the head of the queue, uncharged.
+        The counting workspace is already released, so the batch holds nothing while it waits
+        for other batches to free memory; with nothing else running, the first request runs
+        whatever its size. Called with the lock held."""
+        self._reserved -= sum(r.charge for r in batch)
+        for r in batch:
+            r.charge = 0
+        while True:
+            free = self.budget - self._reserved
+            run, need = [], 0
+            for r, (nv, nf, _) in zip(batch, sizes):
+                f = r.footprint(nv, nf)
+                if need + f > free:
+                    break
+                run.append(r)
+                need += f
+            if run or self._reserved == 0:
+                break
+            self._cv.wait()
+        if not run:
+            run, need = batch[:1], batch[0].footprint(*sizes[0][:2])
+        back = batch[len(run):]
+        self._reserved += need
+        for r, (nv, nf, _) in zip(run, sizes):
+            r.charge = r.footprint(nv, nf)
+        self._queue.extendleft(reversed(back))
+        self.peak_reserved = max(self.peak_reserved, self._reserved)
+        self._cv.notify_all()
+        return run
+
     def _release(self, reqs):
         with self._cv:
             self._reserved -= sum(r.charge for r in reqs)
+            for r in reqs:
+                r.charge = 0
             self._cv.notify_all()
 
     def _work(self):
@@ -149,8 +204,16 @@ class Server:
                         return
                     self._cv.wait()
             try:
+                first = batch[0]
                 with torch.cuda.device(self.device), torch.cuda.stream(stream):
-                    results = self._run(ex, batch)
+                    counted = ex.count([r.sdf for r in batch], [r.algo for r in batch], first.isovalue,
+                                       first.normalize)
+                n_all = len(batch)
+                with self._cv:
+                    batch = self._fit(batch, counted.sizes)
+                with torch.cuda.device(self.device), torch.cuda.stream(stream):
+                    # the counting pass is reused unless part of the batch went back to the queue
+                    results = self._run(ex, batch, counted if len(batch) == n_all else None)
                     stream.synchronize()
                 now = time.perf_counter()
                 with self._cv:
@@ -167,20 +230,13 @@ class Server:
             finally:
                 self._release(batch)
 
-    def _run(self, ex, batch):
+    def _run(self, ex, batch, counted=None):
         first = batch[0]
         want = [r.upstream is not None for r in batch]
         sdfs = [r.sdf.detach().requires_grad_(w) for r, w in zip(batch, want)]
         defs = [None if r.deform is None else r.deform.detach().requires_grad_(w) for r, w in zip(batch, want)]
         outs = ex(sdfs, [r.algo for r in batch], defs, isovalue=first.isovalue, normalize=first.normalize,
-                  return_quads=first.return_quads)
-        # the count pass fixed the output sizes: charge those instead of the estimate
-        with self._cv:
-            for r, (v, f) in zip(batch, outs):
-                ws, _, gr = r.sizes()
-                exact = ws + v.numel() * v.element_size() + f.numel() * f.element_size() + gr
-                self._reserved += exact - r.charge
-                r.charge = exact
+                  return_quads=first.return_quads, counted=counted)
         grads = [None] * len(batch)
         live = [i for i, w in enumerate(want) if w and outs[i][0].shape[0]]
         if live:

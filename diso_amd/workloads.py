@@ -70,6 +70,29 @@ def family_grid(kind, n, rng):
 FAMILIES = ("sphere", "round_box", "blobs", "noise")
 
 
+def _family_torch(kind, n, u, ud, device):
+    """One float64 field of `kind` on an n^3 grid (torch, on `device`); `u` draws the shape
+    parameters (CPU generator), `ud` the noise field (device generator)."""
+    import torch
+
+    a = torch.linspace(-0.5, 0.5, n, dtype=torch.float64, device=device)
+    p = torch.stack(torch.meshgrid(a, a, a, indexing="ij"), -1)
+    if kind == "sphere":
+        return p.norm(dim=-1) - (0.2 + 0.25 * float(u(1)))
+    if kind == "round_box":
+        b = (0.2 + 0.2 * u(3)).to(device)
+        q = p.abs() - b + 0.1
+        return q.clamp_min(0).norm(dim=-1) + q.max(-1)[0].clamp_max(0) - 0.1
+    if kind == "blobs":
+        c = (u(8, 3) * 0.6 - 0.3).to(device)
+        sg = (0.05 + 0.1 * u(8)).to(device)
+        f = torch.zeros(p.shape[:3], dtype=torch.float64, device=device)
+        for k in range(8):
+            f += torch.exp(-((p - c[k]) ** 2).sum(-1) / (2 * sg[k] ** 2))
+        return f - 0.5
+    return ud(n, n, n) * 2 - 1
+
+
 def uneven_batch_torch(count=64, seed=0, sizes=(32, 48, 64, 96, 128, 192, 256), deform_amp=0.3, device="cuda",
                        dtype=None):
     """The uneven batch workload, generated on the device with torch (numpy would take minutes
@@ -89,24 +112,41 @@ def uneven_batch_torch(count=64, seed=0, sizes=(32, 48, 64, 96, 128, 192, 256), 
     for i in range(count):
         n = int(sizes[int(torch.randint(len(sizes), (1,), generator=g))])
         kind = FAMILIES[int(torch.randint(len(FAMILIES), (1,), generator=g))]
-        a = torch.linspace(-0.5, 0.5, n, dtype=torch.float64, device=device)
-        p = torch.stack(torch.meshgrid(a, a, a, indexing="ij"), -1)
-        if kind == "sphere":
-            f = p.norm(dim=-1) - (0.2 + 0.25 * float(u(1)))
-        elif kind == "round_box":
-            b = (0.2 + 0.2 * u(3)).to(device)
-            q = p.abs() - b + 0.1
-            f = q.clamp_min(0).norm(dim=-1) + q.max(-1)[0].clamp_max(0) - 0.1
-        elif kind == "blobs":
-            c = (u(8, 3) * 0.6 - 0.3).to(device)
-            sg = (0.05 + 0.1 * u(8)).to(device)
-            f = torch.zeros(p.shape[:3], dtype=torch.float64, device=device)
-            for k in range(8):
-                f += torch.exp(-((p - c[k]) ** 2).sum(-1) / (2 * sg[k] ** 2))
-            f -= 0.5
-        else:
-            f = ud(n, n, n) * 2 - 1
+        f = _family_torch(kind, n, u, ud, device)
         d = (ud(n, n, n, 3) * 2 - 1) * deform_amp
         out.append((f.to(dtype).contiguous(), d.to(dtype).contiguous(), "mc" if i % 2 == 0 else "dmc", kind, n))
-        del p
+        del f
     return out
+
+
+def request_stream(count, rate, seed=0, pool=48, lo=32, hi=384, deform_amp=0.3, backward_frac=0.3, device="cuda",
+                   dtype=None):
+    """The serving workload: `count` requests arriving as a Poisson process of `rate` per second.
+    Each request takes a grid from a pool of `pool` grids with N log-uniform over [lo, hi] and
+    the family uniform as in the uneven batch (with deform U(-deform_amp, deform_amp)); the
+    algorithm is MC or DMC with probability 1/2 and a backward (upstream N(0,1)) is requested
+    with probability `backward_frac`. Returns (pool [(sdf, deform, family, n)], requests
+    [(arrival_s, pool_index, algo, backward)])."""
+    import math
+
+    import torch
+
+    dtype = dtype or torch.float32
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    gd = torch.Generator(device=device).manual_seed(seed + 1)
+    u = lambda *shape: torch.rand(*shape, generator=g, dtype=torch.float64)
+    ud = lambda *shape: torch.rand(*shape, generator=gd, dtype=torch.float64, device=device)
+    grids = []
+    for _ in range(pool):
+        n = int(round(math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * float(u(1)))))
+        kind = FAMILIES[int(torch.randint(len(FAMILIES), (1,), generator=g))]
+        f = _family_torch(kind, n, u, ud, device)
+        d = (ud(n, n, n, 3) * 2 - 1) * deform_amp
+        grids.append((f.to(dtype).contiguous(), d.to(dtype).contiguous(), kind, n))
+        del f
+    gaps = -torch.log1p(-u(count)) / rate
+    arrivals = torch.cumsum(gaps, 0).tolist()
+    pick = torch.randint(pool, (count,), generator=g).tolist()
+    algo = ["mc" if float(x) < 0.5 else "dmc" for x in u(count)]
+    bwd = [float(x) < backward_frac for x in u(count)]
+    return grids, list(zip(arrivals, pick, algo, bwd))

@@ -88,3 +88,23 @@ def test_batch_twice_and_nan():
     bad[3, 4, 5] = float("nan")
     with pytest.raises(RuntimeError, match="grid 1"):
         ex([g1, bad], ["mc", "mc"])
+
+
+def test_batch_reuses_a_counting_pass():
+    """count() gives the sizes the extraction produces; a call with its handle skips the pass and
+    gives the same results; a handle from other grids or settings is refused."""
+    ex = diso_amd.BatchExtractor()
+    grids = [torch.tensor(inputs(d, k, i, np.float32, False)[0], device=DEV)
+             for i, (d, k) in enumerate([((20, 21, 22), "sphere"), ((33, 12, 40), "noise"), ((8, 8, 8), "blobs")])]
+    algos = ["mc", "dmc", "dmc"]
+    ref = ex(grids, algos, return_quads=True)
+    c = ex.count(grids, algos)
+    assert [(n_v, n_f) for n_v, n_f, _ in c.sizes] == [(v.shape[0], f.shape[0]) for v, f in ref]
+    outs = ex(grids, algos, return_quads=True, counted=c)
+    for (v, f), (pv, pf) in zip(outs, ref):
+        assert torch.equal(v, pv) and torch.equal(f, pf)
+    c = ex.count(grids, algos)
+    with pytest.raises(ValueError, match="counted"):
+        ex(grids, ["mc", "mc", "dmc"], counted=c)
+    with pytest.raises(ValueError, match="counted"):
+        ex(grids[:2], algos[:2], counted=c)

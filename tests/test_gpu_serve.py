@@ -59,7 +59,8 @@ def test_served_results_equal_alone(budget):
         results = [r.future.result(timeout=120) for r in reqs]
         assert srv.batches >= 1 and len(srv.latencies) == len(reqs)
         if budget < (64 << 20):
-            assert srv.peak_reserved <= max(budget, max(r.estimate() for r in reqs))
+            fp = [r.footprint(res[0].shape[0], res[1].shape[0]) for r, res in zip(reqs, results)]
+            assert srv.peak_reserved <= max(budget, max(fp))
         assert srv.latency_ms(50) is not None and srv.latency_ms(99) >= srv.latency_ms(50)
     for res, a in zip(results, alone):
         v, f, gs, gd = _alone(*a)
