@@ -25,6 +25,7 @@ import torch
 
 from . import _lib
 from .batch import BatchExtractor
+from .engine import Extractor, split_quads
 
 __all__ = ["Request", "Server"]
 
@@ -73,13 +74,14 @@ class Request:
 
 class Server:
     def __init__(self, dtype=torch.float32, device="cuda:0", budget_bytes=200 << 30, max_batch=64, workers=2,
-                 reference_compat=False, max_batch_voxels=1 << 22):
+                 reference_compat=False, max_batch_voxels=1 << 20):
         self.dtype = dtype
         self.device = torch.device(device)
         self.budget = int(budget_bytes)
         self.max_batch = int(max_batch)
         # A batch runs pass by pass over all its grids, so its inputs should stay cache-resident
-        # between the passes: grids are batched up to this many voxels (a larger one runs alone).
+        # between the passes: grids are batched up to this many voxels; a request that reaches it
+        # alone runs through the per-grid engine.
         self.max_batch_voxels = int(max_batch_voxels)
         self.reference_compat = reference_compat
         self._queue = collections.deque()
@@ -133,8 +135,10 @@ class Server:
         take = [self._queue.popleft()]
         used = first.workspace()
         voxels = first.sdf.numel()
-        keep = collections.deque()
-        while self._queue and len(take) < self.max_batch and voxels < self.max_batch_voxels:
+        skipped = []
+        # look at a bounded window of the queue (a long queue costs nothing per admission)
+        while (self._queue and len(take) < self.max_batch and voxels < self.max_batch_voxels
+               and len(skipped) < 2 * self.max_batch):
             r = self._queue.popleft()
             if (r.key() == first.key() and used + r.workspace() <= free
                     and voxels + r.sdf.numel() <= self.max_batch_voxels):
@@ -142,9 +146,8 @@ class Server:
                 used += r.workspace()
                 voxels += r.sdf.numel()
             else:
-                keep.append(r)
-        keep.extend(self._queue)
-        self._queue = keep
+                skipped.append(r)
+        self._queue.extendleft(reversed(skipped))
         for r in take:
             r.charge = r.workspace()
         self._reserved += used
@@ -193,6 +196,7 @@ class Server:
     def _work(self):
         stream = torch.cuda.Stream(device=self.device)  # default priority
         ex = BatchExtractor(self.dtype, reference_compat=self.reference_compat)
+        single = {"mc": Extractor(_lib.MC, self.dtype), "dmc": Extractor(_lib.DMC, self.dtype)}
         while True:
             with self._cv:
                 while True:
@@ -204,6 +208,12 @@ class Server:
                         return
                     self._cv.wait()
             try:
+                if len(batch) == 1:
+                    with torch.cuda.device(self.device), torch.cuda.stream(stream):
+                        results = [self._run_one(single[batch[0].algo], batch[0])]
+                        stream.synchronize()
+                    self._finish(batch, results)
+                    continue
                 first = batch[0]
                 with torch.cuda.device(self.device), torch.cuda.stream(stream):
                     counted = ex.count([r.sdf for r in batch], [r.algo for r in batch], first.isovalue,
@@ -215,20 +225,55 @@ class Server:
                     # the counting pass is reused unless part of the batch went back to the queue
                     results = self._run(ex, batch, counted if len(batch) == n_all else None)
                     stream.synchronize()
-                now = time.perf_counter()
-                with self._cv:
-                    self.batches += 1
-                    for r in batch:
-                        r.t_done = now
-                        self.latencies.append(now - r.t_submit)
-                for r, res in zip(batch, results):
-                    r.future.set_result(res)
+                self._finish(batch, results)
             except Exception as e:  # reported to every request of the batch
                 for r in batch:
                     if not r.future.done():
                         r.future.set_exception(e)
             finally:
                 self._release(batch)
+
+    def _finish(self, batch, results):
+        now = time.perf_counter()
+        with self._cv:
+            self.batches += 1
+            for r in batch:
+                r.t_done = now
+                self.latencies.append(now - r.t_submit)
+        for r, res in zip(batch, results):
+            r.future.set_result(res)
+
+    def _run_one(self, ex, r):
+        """A request alone, through the per-grid engine (the passes DiffMC / DiffDMC run, without
+        autograd), admitted on its exact counts."""
+        dmc = r.algo == "dmc"
+        flags = _lib.FLAG_VIRTUAL_PAD | _lib.FLAG_SHIFT | _lib.FLAG_INDEX64
+        if r.normalize:
+            flags |= _lib.FLAG_NORMALIZE
+        if self.reference_compat and dmc:
+            flags |= _lib.FLAG_REFERENCE_COMPAT
+        grid = r.sdf.detach().contiguous()
+        deform = None if r.deform is None else r.deform.detach().contiguous()
+        counted = ex.count(grid, r.isovalue, flags)
+        if counted.has_nan:
+            raise RuntimeError("diso_amd: sdf grid contains NaN values")
+        empty = counted.empty_grid
+        nv, nf = (0, 0) if empty else (counted.n_verts, counted.n_faces)
+        with self._cv:
+            self._fit([r], [(nv, nf, counted.flags)])
+        if empty:  # diso/__init__.py:49-50,103-104
+            return (torch.zeros((0, 3), dtype=self.dtype, device=grid.device),
+                    torch.zeros((0, 4 if dmc else 3), dtype=torch.int32, device=grid.device))
+        verts, faces = ex.emit(grid, deform, counted, flags)
+        if dmc and not r.return_quads:
+            faces = split_quads(verts, faces)
+        if r.upstream is None or nv == 0:
+            return verts, faces
+        up = r.upstream(nv) if callable(r.upstream) else r.upstream
+        gs = torch.empty_like(grid)
+        gd = None if deform is None else torch.empty_like(deform)
+        ex.backward(grid, deform, counted, flags, up.to(self.dtype), gs, gd)
+        return verts, faces, gs, gd
 
     def _run(self, ex, batch, counted=None):
         first = batch[0]
