@@ -1980,8 +1980,22 @@ int cu_count() {
     return cus[dev];
 }
 
-int64_t count_blocks(int64_t nseg) {
-    int64_t nb = (int64_t)cu_count() * g_params.count_per_cu;
+// Count blocks of one CU that can be resident at once for count-kernel instance `kernel` (the
+// single-pass scan wants every block resident; queried once per instance).
+template <auto Kernel>
+int count_resident_per_cu() {
+    static int n = 0;
+    if (n == 0) {
+        int r = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, Kernel, kCountBlock, 0) != hipSuccess || r < 1) r = 1;
+        n = r;
+    }
+    return n;
+}
+
+int64_t count_blocks(int64_t nseg, int resident_per_cu) {
+    const int per_cu = g_params.count_per_cu < resident_per_cu ? g_params.count_per_cu : resident_per_cu;
+    int64_t nb = (int64_t)cu_count() * per_cu;
     const int64_t min_nb = (nseg + (1ll << 20) - 1) >> 20;  // <= 2^20 segments per block keeps the
     if (nb < min_nb) nb = min_nb;                          // aggregate fields from overflowing
     if (nb > kMaxCountBlocks) nb = kMaxCountBlocks;
@@ -2006,16 +2020,18 @@ int launch_count(int algo, const void *sdf, double iso, const Geo &g, void *seg,
     CountSrc<S> src{};
     src.in = make_in<S>(sdf, nullptr, iso, g);
     src.nseg = g.nseg;
-    const int64_t nb = count_blocks(g.nseg);
     (void)ctl_bytes;
     SegBase *sb = (SegBase *)seg;
     uint16_t *info = reinterpret_cast<uint16_t *>(sb + g.nseg + 2);
-    if (algo == DISO_MC)
+    if (algo == DISO_MC) {
+        const int64_t nb = count_blocks(g.nseg, count_resident_per_cu<count_kernel<S, DISO_MC, false>>());
         hipLaunchKernelGGL((count_kernel<S, DISO_MC, false>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, src,
                            (const BatchGrid<S> *)nullptr, sb, info, (uint8_t *)ctl, nb);
-    else
+    } else {
+        const int64_t nb = count_blocks(g.nseg, count_resident_per_cu<count_kernel<S, DISO_DMC, false>>());
         hipLaunchKernelGGL((count_kernel<S, DISO_DMC, false>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, src,
                            (const BatchGrid<S> *)nullptr, sb, info, (uint8_t *)ctl, nb);
+    }
     return check_hip(hipGetLastError(), "count kernel launch");
 }
 
@@ -2134,9 +2150,23 @@ int check_common(int algo, int dtype, int64_t X, int64_t Y, int64_t Z, int flags
     return DISO_OK;
 }
 
+// Pinned staging for the totals copy, one per host thread, kept for the life of the process (a
+// copy into pinned memory completes without a bounce through a pageable buffer).
+SegBase *pinned_totals() {
+    thread_local SegBase *p = nullptr;
+    if (!p) {
+        void *q = nullptr;
+        if (hipHostMalloc(&q, 2 * sizeof(SegBase), hipHostMallocDefault) == hipSuccess) p = (SegBase *)q;
+    }
+    return p;
+}
+
 int read_totals(const Geo &g, const void *seg, diso_totals_t *t, hipStream_t st) {
-    SegBase tot[2];
-    int rc = check_hip(hipMemcpyAsync(tot, (const SegBase *)seg + g.nseg, sizeof(tot), hipMemcpyDeviceToHost, st),
+    SegBase local[2];
+    SegBase *tot = pinned_totals();
+    if (!tot) tot = local;
+    int rc = check_hip(hipMemcpyAsync(tot, (const SegBase *)seg + g.nseg, 2 * sizeof(SegBase), hipMemcpyDeviceToHost,
+                                      st),
                        "totals copy");
     if (rc) return rc;
     rc = check_hip(hipStreamSynchronize(st), "stream synchronize");
@@ -2248,6 +2278,50 @@ struct Group {
     int64_t blocks() const { return rf.back(); }
 };
 
+// Pinned host buffers of one host thread, kept for the life of the process: `up` stages
+// host-to-device copies (reused once the copy recorded in `ev` has run, so the caller never waits
+// for the device), `down` receives device-to-host copies.
+struct Staging {
+    void *up = nullptr, *down = nullptr;
+    size_t up_cap = 0, down_cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+Staging &staging() {
+    thread_local Staging s;
+    return s;
+}
+
+int grow_pinned(void **p, size_t *cap, size_t n) {
+    if (*cap >= n) return DISO_OK;
+    if (*p) check_hip(hipHostFree(*p), "pinned free");
+    *p = nullptr;
+    *cap = 0;
+    const size_t want = n < 65536 ? 65536 : n + n / 2;
+    int rc = check_hip(hipHostMalloc(p, want, hipHostMallocDefault), "pinned allocation");
+    if (rc) return rc;
+    *cap = want;
+    return DISO_OK;
+}
+
+// Copies n host bytes to `dst` on `st` through the pinned staging buffer, without waiting.
+int stage_copy(void *dst, const void *src, size_t n, hipStream_t st) {
+    Staging &S = staging();
+    int rc;
+    if (S.pending) {  // the previous copy from the buffer must have run before it is reused
+        rc = check_hip(hipEventSynchronize(S.ev), "staging wait");
+        if (rc) return rc;
+        S.pending = false;
+    }
+    if (!S.ev && (rc = check_hip(hipEventCreateWithFlags(&S.ev, hipEventDisableTiming), "staging event"))) return rc;
+    if ((rc = grow_pinned(&S.up, &S.up_cap, n))) return rc;
+    memcpy(S.up, src, n);
+    if ((rc = check_hip(hipMemcpyAsync(dst, S.up, n, hipMemcpyHostToDevice, st), "descriptor copy"))) return rc;
+    if ((rc = check_hip(hipEventRecord(S.ev, st), "staging event"))) return rc;
+    S.pending = true;
+    return DISO_OK;
+}
+
 // Host staging of descriptors + group tables, copied into the workspace in one transfer.
 template <typename S>
 int upload_batch(void *ws, const BatchLayout &L, const std::vector<BatchGrid<S>> &d, const Group *groups, int ng,
@@ -2259,10 +2333,7 @@ int upload_batch(void *ws, const BatchLayout &L, const std::vector<BatchGrid<S>>
         memcpy(base, groups[k].gl.data(), groups[k].gl.size() * 4);
         memcpy(base + align256((int64_t)L.n * 4), groups[k].rf.data(), groups[k].rf.size() * 8);
     }
-    int rc = check_hip(hipMemcpyAsync(ws, h.data(), h.size(), hipMemcpyHostToDevice, st), "batch descriptor copy");
-    if (rc) return rc;
-    // the staging buffer is pageable and goes out of scope: wait for the copy
-    return check_hip(hipStreamSynchronize(st), "batch descriptor copy");
+    return stage_copy(ws, h.data(), h.size(), st);
 }
 
 template <typename S>
@@ -2296,7 +2367,7 @@ int batch_count(diso_grid_t *grids, const BatchLayout &L, void *ws, void *ctl, h
     src.gflags = (uint32_t *)((uint8_t *)ctl + align256(ctl_bytes_for(Geo{})));
     src.ngrids = L.n;
     src.nseg = L.nseg;
-    const int64_t nb = count_blocks(L.nseg);
+    const int64_t nb = count_blocks(L.nseg, count_resident_per_cu<count_kernel<S, DISO_MC, true>>());
     hipLaunchKernelGGL((count_kernel<S, DISO_MC, true>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, src,
                        (const BatchGrid<S> *)P.desc, P.seg, P.info, (uint8_t *)ctl, nb);
     rc = check_hip(hipGetLastError(), "batch count kernel launch");
@@ -2305,12 +2376,14 @@ int batch_count(diso_grid_t *grids, const BatchLayout &L, void *ws, void *ctl, h
                        L.nseg, P.seg, src.gflags, P.totals);
     rc = check_hip(hipGetLastError(), "batch totals kernel launch");
     if (rc) return rc;
-    std::vector<diso_totals_t> t(L.n);
-    rc = check_hip(hipMemcpyAsync(t.data(), P.totals, L.n * sizeof(diso_totals_t), hipMemcpyDeviceToHost, st),
-                   "batch totals copy");
+    Staging &stg = staging();
+    const size_t tb = (size_t)L.n * sizeof(diso_totals_t);
+    if ((rc = grow_pinned(&stg.down, &stg.down_cap, tb))) return rc;
+    rc = check_hip(hipMemcpyAsync(stg.down, P.totals, tb, hipMemcpyDeviceToHost, st), "batch totals copy");
     if (rc) return rc;
     rc = check_hip(hipStreamSynchronize(st), "stream synchronize");
     if (rc) return rc;
+    const diso_totals_t *t = (const diso_totals_t *)stg.down;
     for (int i = 0; i < L.n; ++i) {
         grids[i].totals = t[i];
         if (t[i].flags & DISO_TOTAL_SCAN_TIMEOUT) return fail(DISO_E_TIMEOUT, "single-pass scan did not complete");
