@@ -296,7 +296,7 @@ struct Win {
                 for (int j = 0; j < BNY; ++j)
 #pragma unroll
                     for (int k = 0; k < BNZ; ++k) v[BX0 - X0 + i][BY0 - Y0 + j][BZ0 - Z0 + k] = p[i * sx + j * sy + k];
-        } else {
+        } else if (NX * NY > 4) {  // wide windows: per sample (fewer live scalar registers)
 #pragma unroll
             for (int i = 0; i < BNX; ++i)
 #pragma unroll
@@ -304,6 +304,29 @@ struct Win {
 #pragma unroll
                     for (int k = 0; k < BNZ; ++k)
                         v[BX0 - X0 + i][BY0 - Y0 + j][BZ0 - Z0 + k] = sample(in, px + BX0 + i, py + BY0 + j, pz + BZ0 + k);
+        } else {
+            // rows outside the stored grid read as the pad value (a wave-uniform test); inside a
+            // row, lanes whose z is outside load at a valid address and select the pad value
+            const int x0 = px + BX0 - g.off, y0 = py + BY0 - g.off;
+            const int64_t r0 = ((int64_t)x0 * g.n1 + y0) * g.n2;  // offset of row (x0, y0), maybe outside
+#pragma unroll
+            for (int i = 0; i < BNX; ++i)
+#pragma unroll
+                for (int j = 0; j < BNY; ++j) {
+                    if ((unsigned)(x0 + i) < (unsigned)g.n0 && (unsigned)(y0 + j) < (unsigned)g.n1) {
+                        const S *p = in.sdf + (r0 + i * sx + j * sy);
+#pragma unroll
+                        for (int k = 0; k < BNZ; ++k) {
+                            const int z = pz + BZ0 + k - g.off;
+                            const bool inz = (unsigned)z < (unsigned)g.n2;
+                            const S a = p[inz ? z : 0];
+                            v[BX0 - X0 + i][BY0 - Y0 + j][BZ0 - Z0 + k] = inz ? a : in.pad;
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < BNZ; ++k) v[BX0 - X0 + i][BY0 - Y0 + j][BZ0 - Z0 + k] = in.pad;
+                    }
+                }
         }
     }
     __device__ __forceinline__ S operator()(int dx, int dy, int dz) const { return v[dx - X0][dy - Y0][dz - Z0]; }
