@@ -241,11 +241,13 @@ __device__ __forceinline__ int64_t seg_index(const Geo &g, int px, int py, int c
 // (segment-major: word s * 64 + lane), read by the later passes instead of re-deriving it from
 // the samples: bits 0-7 the code of the cell anchored at the position (MC: Bourke corner order;
 // DMC: Morton corner order after the face-neighbour rule), bits 8-10 the crossing mask of the
-// three edges the position owns (+x, +y, +z), bit 11 whether its own sample is >= iso. Bits 0-10
-// are 0 where no cell is anchored.
+// three edges the position owns (+x, +y, +z), bit 11 whether its own sample is >= iso, bits 12-14
+// the cell's face count (MC: triangles, 0..5; DMC: patches, 0..4). Bits 0-10 and 12-14 are 0
+// where no cell is anchored.
 __device__ __forceinline__ int info_code(uint32_t w) { return (int)(w & 0xffu); }
 __device__ __forceinline__ int info_mask(uint32_t w) { return (int)((w >> 8) & 7u); }
 __device__ __forceinline__ int info_inside(uint32_t w) { return (int)((w >> 11) & 1u); }
+__device__ __forceinline__ int info_count(uint32_t w) { return (int)((w >> 12) & 7u); }
 
 __device__ __forceinline__ const uint16_t *info_of(const SegBase *seg, const Geo &g) {
     return reinterpret_cast<const uint16_t *>(seg + g.nseg + 2);
@@ -508,7 +510,7 @@ __device__ __forceinline__ void count_position(const In<S> &in, int ALGO, const 
         if (code != 0 && code != 255) code = dmc_good_code(in, cases, code, px, py, pz);
         *nf = DISO_DMC_NPATCH(cases[code]);
     }
-    *info |= (uint32_t)code | ((uint32_t)m << 8);
+    *info |= (uint32_t)code | ((uint32_t)m << 8) | ((uint32_t)*nf << 12);
 }
 
 // Wave sum of a per-lane value in [0, 7] (three ballots).
@@ -1123,16 +1125,16 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const int cur = info_code(d.cur[2 * i + j]), prev = info_code(d.prev[2 * i + j]);
+            const uint32_t cur = d.cur[2 * i + j], prev = d.prev[2 * i + j];
             int t;
-            const int b = wave_excl_small(DISO_DMC_NPATCH(kDmcCase[cur]), &t);
+            const int b = wave_excl_small(info_count(cur), &t);
             if (lane == 0) L.rowb[2 * i + j] = (int32_t)(d.rowf[2 * i + j] - pabs);
-            L.code[4 * i + 2 * j][lane] = (uint8_t)cur;
+            L.code[4 * i + 2 * j][lane] = (uint8_t)info_code(cur);
             L.pre[4 * i + 2 * j][lane] = (int16_t)b;
-            L.code[4 * i + 2 * j + 1][lane] = (uint8_t)prev;
-            L.pre[4 * i + 2 * j + 1][lane] = (int16_t)(b - DISO_DMC_NPATCH(kDmcCase[prev]));
+            L.code[4 * i + 2 * j + 1][lane] = (uint8_t)info_code(prev);
+            L.pre[4 * i + 2 * j + 1][lane] = (int16_t)(b - info_count(prev));
         }
-    const int np = DISO_DMC_NPATCH(kDmcCase[info_code(w0)]);
+    const int np = info_count(w0);
     int tq, tp;
     const int pq = wave_excl_small(popc3(m00), &tq);
     const int pp = wave_excl_small(np, &tp);
