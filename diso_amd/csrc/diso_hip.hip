@@ -788,6 +788,33 @@ __device__ __forceinline__ void store_index(void *__restrict__ faces, int64_t i,
         reinterpret_cast<int32_t *>(faces)[i] = (int32_t)v;
 }
 
+// Quad q = base + rel[0..3], as four int64 or int32 indices; `vec`: `faces` is 16-byte aligned,
+// so the quad goes out in 16-byte stores.
+typedef long long i64x2 __attribute__((ext_vector_type(2)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_quad(void *__restrict__ faces, int64_t q, int64_t base, const int32_t *rel,
+                                           int flags, bool vec) {
+    if (flags & DISO_FLAG_INDEX64) {
+        int64_t *f = reinterpret_cast<int64_t *>(faces) + 4 * q;
+        if (vec) {
+            reinterpret_cast<i64x2 *>(f)[0] = i64x2{base + rel[0], base + rel[1]};
+            reinterpret_cast<i64x2 *>(f)[1] = i64x2{base + rel[2], base + rel[3]};
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[j] = base + rel[j];
+        }
+    } else {
+        int32_t *f = reinterpret_cast<int32_t *>(faces) + 4 * q;
+        if (vec) {
+            *reinterpret_cast<i32x4 *>(f) = i32x4{(int32_t)(base + rel[0]), (int32_t)(base + rel[1]),
+                                                  (int32_t)(base + rel[2]), (int32_t)(base + rel[3])};
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[j] = (int32_t)(base + rel[j]);
+        }
+    }
+}
+
 __device__ __forceinline__ int64_t load_index(const void *__restrict__ faces, int64_t i, int flags) {
     return (flags & DISO_FLAG_INDEX64) ? reinterpret_cast<const int64_t *>(faces)[i]
                                        : (int64_t) reinterpret_cast<const int32_t *>(faces)[i];
@@ -996,9 +1023,8 @@ struct DmcEmitLds {
     int32_t rowb[4];           // first DMC vertex of row (-i,-j), relative to the segment's first
     uint8_t patch[kWave * 4];  // patch items: lane * 4 + patch
     uint16_t quad[kWave * 3];  // quad items: lane * 8 + axis * 2 + leaving
-    uint8_t ex[8][kWave];      // crossing edges of each kind at the lanes before
-    uint8_t kb[kWave];         // bit k: the lane's edge of kind k crosses
-    int16_t kbase[12];         // first slot of each kind
+    int16_t slot[8][kWave];    // slot of the lane's edge of each kind (-1: does not cross)
+    int16_t hslot[4];          // slots of the halo kinds (position z0 + 64)
     uint16_t eitem[kEdgeSlots];  // edge items: lane << 4 | kind
     S ev[3][kEdgeSlots];       // their MC vertices
 };
@@ -1159,12 +1185,13 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
         const bool on = k < 8 ? ((kbits >> k) & 1) : ((hbits >> (k - 8)) & 1);
         const uint64_t bal = __ballot(on);
         const int ex = __popcll(bal & lt);
-        if (k < 8) L.ex[k][lane] = (uint8_t)ex;
-        if (lane == 0) L.kbase[k] = (int16_t)nslot;
         myslot[k] = on ? nslot + ex : -1;
+        if (k < 8)
+            L.slot[k][lane] = (int16_t)myslot[k];
+        else if (lane == kWave - 1)
+            L.hslot[k - 8] = (int16_t)myslot[k];
         nslot += __popcll(bal);
     }
-    L.kb[lane] = (uint8_t)kbits;
     const bool slots = nslot <= kEdgeSlots;  // wave-uniform
     if (slots) {
 #pragma unroll
@@ -1195,14 +1222,14 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
         const int num = __popc(em);
         S sx = S(0), sy = S(0), sz = S(0);
         if (slots) {
-            const int bits = L.kb[src];
             while (em) {  // ascending edge order
                 const int e = __ffs(em) - 1;
                 em &= em - 1;
                 const int v = (int)(kKinds >> (5 * e)) & 31, kd = v & 7, top = v >> 3;
-                int slot = (top && src == kWave - 1) ? L.kbase[8 + ((kd + 1) >> 1)]
-                                                     : L.kbase[kd] + L.ex[kd][src] + (top & (bits >> kd));
-                slot = slot < nslot ? slot : nslot - 1;  // only a NaN sample can make a patch edge not cross
+                // a top edge belongs to the next position (lane src + 1, or the halo after lane 63)
+                int slot = !top ? L.slot[kd][src] : (src == kWave - 1 ? L.hslot[(kd + 1) >> 1] : L.slot[kd][src + 1]);
+                // only a NaN sample can make a patch edge not cross: keep the read inside the table
+                slot = slot < 0 ? 0 : (slot < nslot ? slot : nslot - 1);
 
                 sx += L.ev[0][slot];
                 sy += L.ev[1][slot];
@@ -1223,18 +1250,21 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
         store_vertex(verts, pbase + k, out.vcap, epilogue(V3<S>{sx * r, sy * r, sz * r}, out.flags, nm));
         if (pbase + k < out.vcap) vinfo[pbase + k] = (uint8_t)(num | (pk << 3));
     }
+    const bool vec = ((uintptr_t)faces & 15) == 0;  // a quad's four indices in 16-byte stores
     for (int k = lane; k < tq; k += kWave) {
         const int it = L.quad[k], src = it >> 3, ty = ((it >> 1) & 3) + 3 * (it & 1);
         const int64_t q = qbase + k;
         if (q >= out.fcap) continue;
         const uint32_t stw = kStencil.w[ty];
+        int32_t rel[4];  // corner ids relative to the segment's first DMC vertex
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int ci = (stw >> (3 * j)) & 7, eid = (stw >> (12 + 4 * j)) & 15;
             const int p = DISO_DMC_PATCH_OF(kDmcCase[L.code[ci][src]], eid);
             // dmcEdgeOffset is -1 for a non-crossing edge (never the case on a padded grid)
-            store_index(faces, 4 * q + j, pbase + L.rowb[ci >> 1] + L.pre[ci][src] + (p == 7 ? -1 : p), out.flags);
+            rel[j] = L.rowb[ci >> 1] + L.pre[ci][src] + (p == 7 ? -1 : p);
         }
+        store_quad(faces, q, pbase, rel, out.flags, vec);
     }
 }
 
