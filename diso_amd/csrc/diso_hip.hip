@@ -1014,7 +1014,7 @@ __device__ constexpr uint64_t edge_kind_table() {
     for (int e = 0; e < 12; ++e) t |= (uint64_t)(edge_kind(e) | (eoz(e) << 3)) << (5 * e);
     return t;
 }
-constexpr int kEdgeSlots = 320;  // MC vertices kept per segment; beyond, patches recompute theirs
+constexpr int kEdgeSlots = 255;  // MC vertices kept per segment; beyond, patches recompute theirs
 
 template <typename S>
 struct DmcEmitLds {
@@ -1023,8 +1023,8 @@ struct DmcEmitLds {
     int32_t rowb[4];           // first DMC vertex of row (-i,-j), relative to the segment's first
     uint8_t patch[kWave * 4];  // patch items: lane * 4 + patch
     uint16_t quad[kWave * 3];  // quad items: lane * 8 + axis * 2 + leaving
-    int16_t slot[8][kWave];    // slot of the lane's edge of each kind (-1: does not cross)
-    int16_t hslot[4];          // slots of the halo kinds (position z0 + 64)
+    uint8_t slot[8][kWave];    // slot of the lane's edge of each kind (0xff: does not cross)
+    uint8_t hslot[4];          // slots of the halo kinds (position z0 + 64)
     uint16_t eitem[kEdgeSlots];  // edge items: lane << 4 | kind
     S ev[3][kEdgeSlots];       // their MC vertices
 };
@@ -1187,9 +1187,9 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
         const int ex = __popcll(bal & lt);
         myslot[k] = on ? nslot + ex : -1;
         if (k < 8)
-            L.slot[k][lane] = (int16_t)myslot[k];
+            L.slot[k][lane] = (uint8_t)myslot[k];
         else if (lane == kWave - 1)
-            L.hslot[k - 8] = (int16_t)myslot[k];
+            L.hslot[k - 8] = (uint8_t)myslot[k];
         nslot += __popcll(bal);
     }
     const bool slots = nslot <= kEdgeSlots;  // wave-uniform
@@ -1229,7 +1229,7 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
                 // a top edge belongs to the next position (lane src + 1, or the halo after lane 63)
                 int slot = !top ? L.slot[kd][src] : (src == kWave - 1 ? L.hslot[(kd + 1) >> 1] : L.slot[kd][src + 1]);
                 // only a NaN sample can make a patch edge not cross: keep the read inside the table
-                slot = slot < 0 ? 0 : (slot < nslot ? slot : nslot - 1);
+                slot = slot < nslot ? slot : (nslot > 0 ? nslot - 1 : 0);
 
                 sx += L.ev[0][slot];
                 sy += L.ev[1][slot];
