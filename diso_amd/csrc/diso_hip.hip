@@ -947,7 +947,7 @@ __device__ __forceinline__ void emit_mc_segment(const In<S> &in, const SegCtx &s
 }
 
 template <typename S, bool DEF>
-__global__ __launch_bounds__(kBlock) void emit_mc_kernel(In<S> in, const SegBase *__restrict__ seg,
+__global__ __launch_bounds__(kBlock, 7) void emit_mc_kernel(In<S> in, const SegBase *__restrict__ seg,
                                                          S *__restrict__ verts, void *__restrict__ faces,
                                                          OutCfg out, Norm<S> nm, int run, int64_t nblocks) {
     __shared__ uint64_t s_case[256];
@@ -1702,7 +1702,7 @@ __global__ void batch_totals_kernel(const BatchGrid<S> *__restrict__ grids, int 
 }
 
 template <typename S, bool DEF>
-__global__ __launch_bounds__(kBlock) void emit_mc_batch_kernel(const BatchGrid<S> *__restrict__ grids,
+__global__ __launch_bounds__(kBlock, 7) void emit_mc_batch_kernel(const BatchGrid<S> *__restrict__ grids,
                                                                const int32_t *__restrict__ gl,
                                                                const int64_t *__restrict__ rf, int m,
                                                                const SegBase *__restrict__ seg,
