@@ -79,7 +79,7 @@ def _run(ex, grid, deform, isovalue, normalize, compat, recompute, empty_k):
     if recompute:
         flags |= _lib.FLAG_RECOMPUTE
     with torch.cuda.device(grid.device):
-        counted = ex.count(grid, iso, flags)
+        counted = ex.count_speculative(grid, deform, iso, flags)
         if counted.has_nan:
             raise RuntimeError("diso_amd: sdf grid contains NaN values")
         if counted.empty_grid:

@@ -21,7 +21,8 @@ class Counted:
     After emission it also holds what a DMC backward needs from the forward: the quads and one
     byte of patch information per DMC vertex."""
 
-    __slots__ = ("seg", "n_verts", "n_faces", "n_crossing", "flags", "dims", "iso", "flags_in", "quads", "vinfo")
+    __slots__ = ("seg", "n_verts", "n_faces", "n_crossing", "flags", "dims", "iso", "flags_in", "quads", "vinfo",
+                 "spec")
 
     def __init__(self, seg, totals, dims, iso, flags_in):
         self.seg = seg
@@ -34,6 +35,7 @@ class Counted:
         self.flags_in = flags_in
         self.quads = None
         self.vinfo = None
+        self.spec = None  # (verts, faces, vinfo) already emitted into buffers these counts fit
 
     @property
     def empty_grid(self):
@@ -63,6 +65,8 @@ class Extractor:
         self.dtype = dtype
         self.code = _DTYPES[dtype]
         self._ctl = {}
+        self._guess = {}  # (device, dims, flags) -> (vertices, faces) of the last call
+        self._side = {}  # device -> (side stream, pinned totals)
         _lib.lib()  # fail loudly now if the HIP library is not there
 
     def check_grid(self, grid, name="grid"):
@@ -99,6 +103,70 @@ class Extractor:
         _lib.check(rc, "count pass")
         return Counted(seg, tot, dims, float(iso), flags)
 
+    def count_speculative(self, grid, deform, iso, flags):
+        """The counting pass, and, when an earlier call on a grid of these dims gives a size
+        estimate, the emission queued right behind it into buffers of that size plus a margin:
+        the host then waits only for the counts (copied on a side stream as soon as the counting
+        kernel is done), so its work until the next launch overlaps the emission instead of
+        leaving the device idle. If the counts fit, `counted.spec` holds the emitted buffers
+        (emit() returns views of them); otherwise emit() runs the emission again at the exact
+        size (the queued one writes nothing when the counts exceed its capacities)."""
+        dims = tuple(int(d) for d in grid.shape)
+        dev = grid.device
+        key = (dev.index, dims, flags)
+        guess = self._guess.get(key)
+        if guess is None:
+            counted = self.count(grid, iso, flags)
+            self._remember(key, counted.n_verts, counted.n_faces)
+            return counted
+        L = _lib.lib()
+        nseg = L.diso_seg_bytes(*dims, flags & ~_lib.FLAG_RECOMPUTE)
+        off = L.diso_totals_offset(*dims, flags)
+        if nseg < 0 or off < 0:
+            _lib.check(min(nseg, off), "workspace sizing")
+        seg = torch.empty(nseg, dtype=torch.uint8, device=dev)
+        ctl = self.ctl_for(dev, dims, flags)
+        main = torch.cuda.current_stream(dev)
+        rc = L.diso_count(self.algo, self.code, _ptr(grid), *dims, float(iso), flags, _ptr(seg), nseg, _ptr(ctl),
+                          ctl.numel(), None, ctypes.c_void_p(main.cuda_stream))
+        _lib.check(rc, "count pass")
+        counted_ev = torch.cuda.Event()
+        counted_ev.record(main)
+        dmc = self.algo == _lib.DMC
+        vcap, fcap = guess[0] + guess[0] // 4 + 1024, guess[1] + guess[1] // 4 + 1024
+        idx = torch.int64 if flags & _lib.FLAG_INDEX64 else torch.int32
+        verts = torch.empty((vcap, 3), dtype=self.dtype, device=dev)
+        faces = torch.empty((fcap, 4 if dmc else 3), dtype=idx, device=dev)
+        vinfo = torch.empty(vcap if dmc else 0, dtype=torch.uint8, device=dev)
+        rc = L.diso_emit(self.algo, self.code, _ptr(grid), _ptr(deform), *dims, float(iso), flags, _ptr(seg), nseg,
+                         _ptr(verts), vcap, _ptr(faces), fcap, _ptr(vinfo), ctypes.c_void_p(main.cuda_stream))
+        _lib.check(rc, "emit pass")
+        side, host = self._side.get(dev.index, (None, None))
+        if side is None:
+            side = torch.cuda.Stream(device=dev)  # default priority
+            host = torch.empty(4, dtype=torch.int64, pin_memory=True)
+            self._side[dev.index] = (side, host)
+        side.wait_event(counted_ev)
+        with torch.cuda.stream(side):
+            host.copy_(seg[off:off + 32].view(torch.int64), non_blocking=True)
+        side.synchronize()
+        crossing, other, tflags, _ = (int(x) for x in host.tolist())
+        tot = _lib.Totals()
+        tot.n_verts, tot.n_faces = (other, crossing) if dmc else (crossing, other)
+        tot.flags, tot.n_crossing = tflags, crossing
+        if tflags & _lib.TOTAL_SCAN_TIMEOUT:
+            raise RuntimeError("diso_amd: count pass failed: single-pass scan did not complete")
+        counted = Counted(seg, tot, dims, float(iso), flags)
+        if tot.n_verts <= vcap and tot.n_faces <= fcap:
+            counted.spec = (verts, faces, vinfo)
+        self._remember(key, tot.n_verts, tot.n_faces)
+        return counted
+
+    def _remember(self, key, n_verts, n_faces):
+        if len(self._guess) >= 64 and key not in self._guess:
+            self._guess.clear()  # many different grids: start over
+        self._guess[key] = (n_verts, n_faces)
+
     def _scratch(self, counted, device):
         n = _lib.lib().diso_scratch_bytes(self.algo, self.code, counted.n_crossing)
         if n < 0:
@@ -111,13 +179,19 @@ class Extractor:
         dmc = self.algo == _lib.DMC
         k = 4 if dmc else 3
         idx = torch.int64 if flags & _lib.FLAG_INDEX64 else torch.int32
-        verts = torch.empty((counted.n_verts, 3), dtype=self.dtype, device=grid.device)
-        faces = torch.empty((counted.n_faces, k), dtype=idx, device=grid.device)
-        vinfo = torch.empty(counted.n_verts if dmc else 0, dtype=torch.uint8, device=grid.device)
-        rc = L.diso_emit(self.algo, self.code, _ptr(grid), _ptr(deform), *counted.dims, counted.iso, flags,
-                         _ptr(counted.seg), counted.seg.numel(), _ptr(verts), counted.n_verts, _ptr(faces),
-                         counted.n_faces, _ptr(vinfo), _stream(grid.device))
-        _lib.check(rc, "emit pass")
+        if counted.spec is not None:  # emitted already (count_speculative): exact-size views
+            v, f, vi = counted.spec
+            counted.spec = None
+            verts, faces = v[:counted.n_verts], f[:counted.n_faces]
+            vinfo = vi[:counted.n_verts] if dmc else vi
+        else:
+            verts = torch.empty((counted.n_verts, 3), dtype=self.dtype, device=grid.device)
+            faces = torch.empty((counted.n_faces, k), dtype=idx, device=grid.device)
+            vinfo = torch.empty(counted.n_verts if dmc else 0, dtype=torch.uint8, device=grid.device)
+            rc = L.diso_emit(self.algo, self.code, _ptr(grid), _ptr(deform), *counted.dims, counted.iso, flags,
+                             _ptr(counted.seg), counted.seg.numel(), _ptr(verts), counted.n_verts, _ptr(faces),
+                             counted.n_faces, _ptr(vinfo), _stream(grid.device))
+            _lib.check(rc, "emit pass")
         if dmc:
             counted.quads, counted.vinfo = faces, vinfo
         if flags & _lib.FLAG_RECOMPUTE:

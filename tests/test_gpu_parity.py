@@ -259,3 +259,36 @@ def test_results_do_not_depend_on_launch_parameters():
     for other in outs[1:]:
         for a, b in zip(outs[0], other):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("algo", [O.MC, O.DMC])
+@pytest.mark.parametrize("recompute", [False, True])
+def test_reused_module_matches_fresh_module(algo, recompute):
+    """A module reused on grids of the same dims emits speculatively right behind its counting
+    pass (into buffers sized from the previous call): counts that fit (views of those buffers),
+    counts that outgrow them (emitted again), an empty grid and the triangle split all give the
+    results of a fresh module, gradients included."""
+    dims = (30, 26, 70)
+    mk = (lambda: diso_amd.DiffMC(recompute_backward=recompute)) if algo == O.MC else \
+        (lambda: diso_amd.DiffDMC(recompute_backward=recompute))
+    reused = mk()
+    seq = [("sphere", 0), ("sphere", 1), ("noise", 2), ("noisy_box", 3), ("empty", 4), ("sphere", 5), ("noise", 6)]
+    for i, (kind, seed) in enumerate(seq):
+        print(f"case reuse algo={algo} recompute={recompute} step={i} kind={kind}", flush=True)
+        if kind == "empty":
+            sdf_np, def_np = np.full(dims, 0.5, np.float32), np.zeros(dims + (3,), np.float32)
+        else:
+            sdf_np, def_np = inputs(dims, kind, seed, np.float32, True)
+        res = []
+        for mod in (reused, mk()):
+            sdf = torch.tensor(sdf_np, device=DEV, requires_grad=True)
+            deform = torch.tensor(def_np, device=DEV, requires_grad=True)
+            kw = {"return_quads": i % 2 == 0} if algo == O.DMC else {}
+            v, f = mod(sdf, deform, **kw)
+            if v.shape[0]:
+                (v * v).sum().backward()
+            res.append((v.detach(), f, sdf.grad, deform.grad))
+        (v1, f1, gs1, gd1), (v2, f2, gs2, gd2) = res
+        assert v1.shape == v2.shape and f1.dtype == f2.dtype and torch.equal(v1, v2) and torch.equal(f1, f2), i
+        if v2.shape[0]:
+            assert torch.equal(gs1, gs2) and torch.equal(gd1, gd2), i
