@@ -292,3 +292,21 @@ def test_reused_module_matches_fresh_module(algo, recompute):
         assert v1.shape == v2.shape and f1.dtype == f2.dtype and torch.equal(v1, v2) and torch.equal(f1, f2), i
         if v2.shape[0]:
             assert torch.equal(gs1, gs2) and torch.equal(gd1, gd2), i
+
+
+@pytest.mark.parametrize("algo", [O.MC, O.DMC])
+@pytest.mark.parametrize("normalize", [True, False])
+def test_fp32_vertices_within_1e6_of_fp64_oracle(algo, normalize):
+    """fp32 extraction against the fp64 oracle on the same (fp32-representable) samples: identical
+    topology (the classification compares the same values) and vertices within 1e-6 of the
+    coordinate scale (max |v|)."""
+    print(f"case fp32 vs fp64 oracle algo={algo} normalize={normalize}", flush=True)
+    sdf_np, def_np = inputs((64, 57, 70), "noisy_box", 9, np.float32, True)
+    ref = O.Extractor(algo, np.float64)
+    rv, rf = ref.forward(sdf_np.astype(np.float64), def_np.astype(np.float64), 0.0, normalize, return_quads=True)
+    kw = {"return_quads": True} if algo == O.DMC else {}
+    v, f = _module(algo, np.float32)(torch.tensor(sdf_np, device=DEV), torch.tensor(def_np, device=DEV),
+                                     normalize=normalize, **kw)
+    assert np.array_equal(f.cpu().numpy(), rf)
+    err = np.abs(v.cpu().numpy().astype(np.float64) - rv).max()
+    assert err <= 1e-6 * np.abs(rv).max(), err
