@@ -39,7 +39,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
-constexpr uint32_t kSpinLimit = 1u << 22;  // bounded look-back wait (~seconds); never expected
+constexpr uint32_t kSpinLimit = 1u << 20;  // bounded look-back wait (well under a second); never expected
 constexpr int kCtlHeader = 64;             // bytes
 
 __constant__ uint64_t kMcCase[256] = {DISO_MC_CASE_VALUES};
@@ -441,23 +441,27 @@ struct Norm {
 // ------------------------------------------------------------------------------------------
 // Pass 1: classify, count, scan (single pass)
 //
-// The grid of segments is split into G contiguous ranges, one per block (G ~ 4 blocks per CU, so
-// all blocks are resident). A block counts its segments (writing each segment's counts into
+// The grid of segments is split into G contiguous ranges (G ~ 4 blocks per CU). A block's range
+// and place in the scan are given by a ticket it takes when it starts (the order in which blocks
+// start), so a block only ever waits on blocks that are already running, whatever else shares the
+// device. A block counts its segments (writing each segment's counts into
 // `seg`), publishes its aggregate as one self-contained 64-bit word, reads the aggregates of all
 // its predecessors in parallel (a look-back over the whole window at once: no chain of inclusive
 // prefixes, so the wait is one memory latency rather than one per predecessor), and finally scans
 // its own segment counts into segment bases. Aggregate words are accessed with agent-scope
 // atomics only, so they are coherent across the XCDs' L2s without fences. Two word buffers
 // alternate between calls; each call clears the one the next call uses (up to the extent that
-// was written), and the last block flips the parity, so no initialisation launch is needed.
+// was written) and resets its ticket counter, and the last block flips the parity, so no
+// initialisation launch is needed.
 //
 // Aggregate word: [63:62] status (1 = published), [61:59] sample flags, [58:29] face / patch
 // count, [28:0] vertex / quad count.
 
 struct CtlHeader {
     uint32_t parity;
-    uint32_t dirty[2];  // words written in each buffer since it was last cleared
-    uint32_t pad[13];
+    uint32_t dirty[2];   // words written in each buffer since it was last cleared
+    uint32_t ticket[2];  // blocks started in the call using each buffer
+    uint32_t pad[11];
 };
 
 constexpr int kCountWaves = 8;
@@ -549,14 +553,12 @@ __global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCou
     __shared__ uint64_t s_case[BATCH ? 2 : 1][256];
     __shared__ int64_t s_v[kCountWaves], s_f[kCountWaves];
     __shared__ uint32_t s_fl[kCountWaves];
-    __shared__ uint32_t s_parity, s_dirty_other;
+    __shared__ uint32_t s_parity, s_dirty_other, s_ticket;
 
     CtlHeader *hdr = reinterpret_cast<CtlHeader *>(ctl);
     uint64_t *words = reinterpret_cast<uint64_t *>(ctl + kCtlHeader);
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     const int64_t nseg = src.nseg;
-    const int64_t b = blockIdx.x;
-    const int64_t lo = b * nseg / nblocks, hi = (b + 1) * nseg / nblocks;
 
     for (int i = tid; i < 256; i += kCountBlock) {
         if (BATCH) {
@@ -567,12 +569,16 @@ __global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCou
         }
     }
     if (tid == 0) {
-        s_parity = __hip_atomic_load(&hdr->parity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_dirty_other =
-            __hip_atomic_load(&hdr->dirty[s_parity ^ 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t par = __hip_atomic_load(&hdr->parity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_parity = par;
+        s_dirty_other = __hip_atomic_load(&hdr->dirty[par ^ 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the ticket's address depends on the parity read, so that read completes first
+        s_ticket = __hip_atomic_fetch_add(&hdr->ticket[par], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     const uint32_t parity = s_parity;
+    const int64_t b = s_ticket;  // this block's range and place in the scan
+    const int64_t lo = b * nseg / nblocks, hi = (b + 1) * nseg / nblocks;
     uint64_t *cur = words + (int64_t)parity * kMaxCountBlocks;
 
     // 1. Count: wave w takes the w-th eighth of the block's segments, in order (its position
@@ -689,6 +695,9 @@ __global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCou
         // Every block has read the parity (it published a word, which this block waited for).
         __hip_atomic_store(&hdr->dirty[parity], (uint32_t)nblocks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&hdr->dirty[parity ^ 1u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // every block of this call has taken its ticket (this one is the last); the next call
+        // takes tickets from the other counter
+        __hip_atomic_store(&hdr->ticket[parity ^ 1u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&hdr->parity, parity ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 
