@@ -46,11 +46,13 @@ def _alone(sdf, deform, algo, backward, seed):
     return v.detach(), f, gs, gd[0] if gd else None
 
 
-@pytest.mark.parametrize("budget", [200 << 30, 4 << 20])
-def test_served_results_equal_alone(budget):
-    print(f"case serve budget={budget}", flush=True)
+@pytest.mark.parametrize("budget,workers", [(200 << 30, 2), (4 << 20, 2), (200 << 30, 4)])
+def test_served_results_equal_alone(budget, workers):
+    """Four workers run counting passes concurrently on four streams (each pass's single-pass scan
+    must not wait on blocks of its own launch that cannot run)."""
+    print(f"case serve budget={budget} workers={workers}", flush=True)
     reqs, alone = _requests(100)
-    with Server(budget_bytes=budget, workers=2, max_batch=4) as srv:
+    with Server(budget_bytes=budget, workers=workers, max_batch=4) as srv:
         threads = [threading.Thread(target=lambda rs=reqs[k::3]: [srv.submit(r) for r in rs]) for k in range(3)]
         for t in threads:
             t.start()
@@ -69,3 +71,37 @@ def test_served_results_equal_alone(budget):
             assert torch.equal(res[2], gs)
             if gd is not None:
                 assert torch.equal(res[3], gd)
+
+
+def test_concurrent_streams_match_sequential():
+    """Regression test of the single-pass scan under concurrency: four threads, each on its own
+    stream, run DiffMC / DiffDMC forward+backward on 96^3 grids at the same time; every result
+    equals the sequential one."""
+    print("case concurrent streams", flush=True)
+    jobs = []
+    for i in range(8):
+        sdf_np, def_np = inputs((96, 96, 96), "noise" if i % 2 else "blobs", 200 + i, np.float32, True)
+        jobs.append((torch.tensor(sdf_np, device=DEV), torch.tensor(def_np, device=DEV), "dmc" if i % 3 else "mc", 300 + i))
+    want = [_alone(s, d, a, True, seed) for s, d, a, seed in jobs]
+    got = [None] * len(jobs)
+    errors = []
+
+    def worker(k):
+        try:
+            st = torch.cuda.Stream()
+            with torch.cuda.stream(st):
+                for rep in range(3):
+                    for i in range(k, len(jobs), 4):
+                        got[i] = _alone(*jobs[i][:3], True, jobs[i][3])
+                st.synchronize()
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for (v, f, gs, gd), (v2, f2, gs2, gd2) in zip(want, got):
+        assert torch.equal(v, v2) and torch.equal(f, f2) and torch.equal(gs, gs2) and torch.equal(gd, gd2)
