@@ -36,6 +36,7 @@ TOTAL_HAS_NAN = 1 << 2
 TOTAL_SCAN_TIMEOUT = 1 << 3
 PARAM_RUN = 0
 PARAM_COUNT_BLOCKS_PER_CU = 1
+PARAM_RECOUNT_SPINS = 2
 
 EXPORTS = [
     "diso_seg_bytes", "diso_ctl_bytes", "diso_scratch_bytes", "diso_count", "diso_emit", "diso_backward",

@@ -441,27 +441,27 @@ struct Norm {
 // ------------------------------------------------------------------------------------------
 // Pass 1: classify, count, scan (single pass)
 //
-// The grid of segments is split into G contiguous ranges (G ~ 4 blocks per CU). A block's range
-// and place in the scan are given by a ticket it takes when it starts (the order in which blocks
-// start), so a block only ever waits on blocks that are already running, whatever else shares the
-// device. A block counts its segments (writing each segment's counts into
+// The grid of segments is split into G contiguous ranges, one per block (G ~ 4 blocks per CU).
+// A block counts its segments (writing each segment's counts into
 // `seg`), publishes its aggregate as one self-contained 64-bit word, reads the aggregates of all
 // its predecessors in parallel (a look-back over the whole window at once: no chain of inclusive
 // prefixes, so the wait is one memory latency rather than one per predecessor), and finally scans
 // its own segment counts into segment bases. Aggregate words are accessed with agent-scope
 // atomics only, so they are coherent across the XCDs' L2s without fences. Two word buffers
 // alternate between calls; each call clears the one the next call uses (up to the extent that
-// was written) and resets its ticket counter, and the last block flips the parity, so no
-// initialisation launch is needed.
+// was written), and the last block flips the parity once every block has published (and so has
+// read the parity), so no initialisation launch is needed. A predecessor that does not publish
+// in time (it may not be running: other launches can fill the slots of the XCD it was dispatched
+// to) is counted by the waiting block itself; only the last block waits for all, and every block
+// it waits for ends in bounded time once dispatched.
 //
 // Aggregate word: [63:62] status (1 = published), [61:59] sample flags, [58:29] face / patch
 // count, [28:0] vertex / quad count.
 
 struct CtlHeader {
     uint32_t parity;
-    uint32_t dirty[2];   // words written in each buffer since it was last cleared
-    uint32_t ticket[2];  // blocks started in the call using each buffer
-    uint32_t pad[11];
+    uint32_t dirty[2];  // words written in each buffer since it was last cleared
+    uint32_t pad[13];
 };
 
 constexpr int kCountWaves = 8;
@@ -543,52 +543,21 @@ struct CountSrc {
     In<S> in;
     uint32_t *gflags;
     int ngrids;
-    int64_t nseg;  // segments of the launch
+    uint32_t recount_spins;  // reads of an unpublished predecessor word before counting its range
+    int64_t nseg;            // segments of the launch
 };
 
-template <typename S, int ALGO, bool BATCH>
-__global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCountWaves / 4) void count_kernel(
-    CountSrc<S> src, const BatchGrid<S> *__restrict__ grids, SegBase *seg, uint16_t *info, uint8_t *ctl,
-    int64_t nblocks) {
-    __shared__ uint64_t s_case[BATCH ? 2 : 1][256];
-    __shared__ int64_t s_v[kCountWaves], s_f[kCountWaves];
-    __shared__ uint32_t s_fl[kCountWaves];
-    __shared__ uint32_t s_parity, s_dirty_other, s_ticket;
-
-    CtlHeader *hdr = reinterpret_cast<CtlHeader *>(ctl);
-    uint64_t *words = reinterpret_cast<uint64_t *>(ctl + kCtlHeader);
-    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
-    const int64_t nseg = src.nseg;
-
-    for (int i = tid; i < 256; i += kCountBlock) {
-        if (BATCH) {
-            s_case[0][i] = kMcCase[i];
-            s_case[BATCH ? 1 : 0][i] = kDmcCase[i];
-        } else {
-            s_case[0][i] = ALGO == DISO_MC ? kMcCase[i] : kDmcCase[i];
-        }
-    }
-    if (tid == 0) {
-        const uint32_t par = __hip_atomic_load(&hdr->parity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_parity = par;
-        s_dirty_other = __hip_atomic_load(&hdr->dirty[par ^ 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // the ticket's address depends on the parity read, so that read completes first
-        s_ticket = __hip_atomic_fetch_add(&hdr->ticket[par], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    const uint32_t parity = s_parity;
-    const int64_t b = s_ticket;  // this block's range and place in the scan
-    const int64_t lo = b * nseg / nblocks, hi = (b + 1) * nseg / nblocks;
-    uint64_t *cur = words + (int64_t)parity * kMaxCountBlocks;
-
-    // 1. Count: wave w takes the w-th eighth of the block's segments, in order (its position
-    // advances without divisions); lane 0 records each segment's counts, every lane its
-    // classification word.
-    int64_t wv = 0, wf = 0;
-    uint32_t wfl = 0;
-    const int64_t wlo = lo + (hi - lo) * wave / kCountWaves, whi = lo + (hi - lo) * (wave + 1) / kCountWaves;
-    // BATCH: the wave's range may span several grids; one pass of the outer loop per grid, with
-    // that grid's parameters held in scalar registers for the inner loop.
+// Counts segments [wlo, whi) (one wave, in order; its position advances without divisions) and
+// adds the wave's totals to (wv, wf, wfl). WRITE: also record each segment's counts (lane 0),
+// every lane's classification word and (BATCH) the grids' sample flags; without WRITE the range
+// is only summed (a block recounting a predecessor's range, see the look-back).
+template <typename S, int ALGO, bool BATCH, bool WRITE>
+__device__ __forceinline__ void count_range(const CountSrc<S> &src, const BatchGrid<S> *__restrict__ grids,
+                                            SegBase *seg, uint16_t *info, const uint64_t (*s_case)[256],
+                                            int64_t wlo, int64_t whi, int lane, int64_t &wv, int64_t &wf,
+                                            uint32_t &wfl) {
+    // BATCH: the range may span several grids; one pass of the outer loop per grid, with that
+    // grid's parameters held in scalar registers for the inner loop.
     int gi = 0;
     if constexpr (BATCH)
         while (gi + 1 < src.ngrids && wlo >= grids[gi + 1].seg_first) ++gi;
@@ -618,11 +587,13 @@ __global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCou
                     ++sp.px;
                 }
             }
-            info[s * kWave + lane] = (uint16_t)iw;
             const int tv = wave_total_small(nv), tf = wave_total_small(nf);
-            if (lane == 0) {
-                seg[s].v = tv;
-                seg[s].f = tf;
+            if constexpr (WRITE) {
+                info[s * kWave + lane] = (uint16_t)iw;
+                if (lane == 0) {
+                    seg[s].v = tv;
+                    seg[s].f = tf;
+                }
             }
             wv += tv;
             wf += tf;
@@ -630,11 +601,55 @@ __global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCou
         }
         wfl |= gfl;
         if constexpr (BATCH) {
-            gfl = wave_or(gfl);
-            if (lane == 0 && gfl) atomicOr(src.gflags + gi, gfl);
+            if constexpr (WRITE) {
+                gfl = wave_or(gfl);
+                if (lane == 0 && gfl) atomicOr(src.gflags + gi, gfl);
+            }
             ++gi;
         }
     }
+}
+
+template <typename S, int ALGO, bool BATCH>
+__global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCountWaves / 4) void count_kernel(
+    CountSrc<S> src, const BatchGrid<S> *__restrict__ grids, SegBase *seg, uint16_t *info, uint8_t *ctl,
+    int64_t nblocks) {
+    __shared__ uint64_t s_case[BATCH ? 2 : 1][256];
+    __shared__ int64_t s_v[kCountWaves], s_f[kCountWaves];
+    __shared__ uint32_t s_fl[kCountWaves];
+    __shared__ uint32_t s_parity, s_dirty_other, s_next;
+    __shared__ uint64_t s_word;
+
+    CtlHeader *hdr = reinterpret_cast<CtlHeader *>(ctl);
+    uint64_t *words = reinterpret_cast<uint64_t *>(ctl + kCtlHeader);
+    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
+    const int64_t nseg = src.nseg;
+    const int64_t b = blockIdx.x;
+    const int64_t lo = b * nseg / nblocks, hi = (b + 1) * nseg / nblocks;
+
+    for (int i = tid; i < 256; i += kCountBlock) {
+        if (BATCH) {
+            s_case[0][i] = kMcCase[i];
+            s_case[BATCH ? 1 : 0][i] = kDmcCase[i];
+        } else {
+            s_case[0][i] = ALGO == DISO_MC ? kMcCase[i] : kDmcCase[i];
+        }
+    }
+    if (tid == 0) {
+        const uint32_t par = __hip_atomic_load(&hdr->parity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_parity = par;
+        s_dirty_other = __hip_atomic_load(&hdr->dirty[par ^ 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t parity = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_parity);
+    uint64_t *cur = words + (int64_t)parity * kMaxCountBlocks;
+
+    // 1. Count: wave w takes the w-th eighth of the block's segments, in order; lane 0 records
+    // each segment's counts, every lane its classification word.
+    int64_t wv = 0, wf = 0;
+    uint32_t wfl = 0;
+    count_range<S, ALGO, BATCH, true>(src, grids, seg, info, s_case, lo + (hi - lo) * wave / kCountWaves,
+                                      lo + (hi - lo) * (wave + 1) / kCountWaves, lane, wv, wf, wfl);
     wfl = wave_or(wfl);
     if (lane == 0) {
         s_v[wave] = wv;
@@ -652,30 +667,40 @@ __global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCou
     }
     if (tid == 0) st_relaxed(cur + b, pack_agg(av, af, afl));
 
-    // 2. Exclusive prefix of this block = sum of all predecessors' aggregates.
+    // 2. Exclusive prefix of this block = sum of all predecessors' aggregates. A predecessor that
+    // has not published within `recount_spins` reads is not waited for any longer: this block
+    // counts that range itself (read-only; the same sums), so no block waits on one that may not
+    // be running (several launches sharing the device fill the XCDs' slots). The last block alone
+    // waits for every predecessor's word: a block publishes after reading the parity, so once all
+    // have published the last block may flip it; every other block ends in bounded time, so the
+    // blocks it waits for are all dispatched.
+    const bool last = b == nblocks - 1;
+    const uint32_t spin_limit = last ? kSpinLimit / (uint32_t)(kMaxCountBlocks / kCountBlock) : src.recount_spins;
     int64_t pv = 0, pf = 0;
     uint32_t pfl = 0;
-    bool timeout = false;
-    for (int64_t j = tid; j < b; j += kCountBlock) {
+    uint32_t miss = 0;  // bit k: predecessor tid + k * kCountBlock did not publish in time
+    for (int64_t j = tid, k = 0; j < b; j += kCountBlock, ++k) {
         uint64_t w;
         uint32_t spins = 0;
         do {
             w = ld_relaxed(cur + j);
-        } while ((w >> 62) == 0 && ++spins < kSpinLimit);
-        timeout |= (w >> 62) == 0;
-        pv += (int64_t)(w & ((1ull << 29) - 1));
-        pf += (int64_t)((w >> 29) & ((1ull << 30) - 1));
-        pfl |= (uint32_t)((w >> 59) & 7);
+        } while ((w >> 62) == 0 && ++spins < spin_limit);
+        if ((w >> 62) == 0) {
+            miss |= 1u << k;
+        } else {
+            pv += (int64_t)(w & ((1ull << 29) - 1));
+            pf += (int64_t)((w >> 29) & ((1ull << 30) - 1));
+            pfl |= (uint32_t)((w >> 59) & 7);
+        }
     }
     pv = wave_sum64(pv);
     pf = wave_sum64(pf);
     pfl = wave_or(pfl);
-    const bool wave_timeout = __ballot(timeout) != 0;
     __syncthreads();  // s_v / s_f reuse
     if (lane == 0) {
         s_v[wave] = pv;
         s_f[wave] = pf;
-        s_fl[wave] = pfl | (wave_timeout ? DISO_TOTAL_SCAN_TIMEOUT : 0u);
+        s_fl[wave] = pfl;
     }
     __syncthreads();
     int64_t ev = 0, ef = 0;
@@ -686,24 +711,62 @@ __global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCou
         ef += s_f[w];
         efl |= s_fl[w];
     }
-    if (b == nblocks - 1 && tid == 0) {
+    // the missing predecessors, lowest first, one at a time by the whole block
+    for (;;) {
+        if (tid == 0) s_next = 0xffffffffu;
+        __syncthreads();
+        if (miss) atomicMin(&s_next, (uint32_t)(tid + __builtin_ctz(miss) * kCountBlock));  // LDS
+        __syncthreads();
+        const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_next);
+        if (j == 0xffffffffu) break;  // block-uniform
+        if (last) efl |= DISO_TOTAL_SCAN_TIMEOUT;  // never expected: the parity may flip early
+        if (tid == (int)(j % kCountBlock)) {  // one read for the whole block: every wave takes the same branch
+            miss &= ~(1u << (j / kCountBlock));
+            s_word = ld_relaxed(cur + j);
+        }
+        __syncthreads();
+        const uint64_t w = s_word;
+        int64_t rv = 0, rf = 0;
+        uint32_t rfl = 0;
+        if (__builtin_amdgcn_readfirstlane((int)(w >> 62)) != 0) {  // published meanwhile
+            rv = (int64_t)(w & ((1ull << 29) - 1));
+            rf = (int64_t)((w >> 29) & ((1ull << 30) - 1));
+            rfl = (uint32_t)((w >> 59) & 7);
+            if (wave != 0) rv = rf = 0, rfl = 0;  // count it once
+        } else {
+            const int64_t jl = (int64_t)j * nseg / nblocks, jh = ((int64_t)j + 1) * nseg / nblocks;
+            count_range<S, ALGO, BATCH, false>(src, grids, seg, info, s_case, jl + (jh - jl) * wave / kCountWaves,
+                                               jl + (jh - jl) * (wave + 1) / kCountWaves, lane, rv, rf, rfl);
+            rfl = wave_or(rfl);
+        }
+        __syncthreads();  // s_v / s_f reuse
+        if (lane == 0) {  // rv, rf are wave-uniform sums
+            s_v[wave] = rv;
+            s_f[wave] = rf;
+            s_fl[wave] = rfl;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w2 = 0; w2 < kCountWaves; ++w2) {
+            ev += s_v[w2];
+            ef += s_f[w2];
+            efl |= s_fl[w2];
+        }
+    }
+    if (last && tid == 0) {
         SegBase *tot = seg + nseg;
         tot[0].v = ev + av;
         tot[0].f = ef + af;
         tot[1].v = (int64_t)(efl | afl);
         tot[1].f = ALGO;
-        // Every block has read the parity (it published a word, which this block waited for).
         __hip_atomic_store(&hdr->dirty[parity], (uint32_t)nblocks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&hdr->dirty[parity ^ 1u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // every block of this call has taken its ticket (this one is the last); the next call
-        // takes tickets from the other counter
-        __hip_atomic_store(&hdr->ticket[parity ^ 1u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&hdr->parity, parity ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 
     // 3. Clear this block's share of the other word buffer for the next call.
     {
-        const int64_t n = s_dirty_other;
+        const int64_t n = (int64_t)__builtin_amdgcn_readfirstlane((int)s_dirty_other);
         const int64_t per = (n + nblocks - 1) / nblocks;
         const int64_t c0 = b * per, c1 = c0 + per < n ? c0 + per : n;
         uint64_t *other = words + (int64_t)(parity ^ 1u) * kMaxCountBlocks;
@@ -2004,6 +2067,7 @@ int check_hip(hipError_t e, const char *what) {
 struct Params {
     int run = 32;
     int count_per_cu = kCountBlocksPerCU;
+    int recount_spins = 8192;
 };
 Params g_params;
 
@@ -2084,6 +2148,7 @@ int launch_count(int algo, const void *sdf, double iso, const Geo &g, void *seg,
     CountSrc<S> src{};
     src.in = make_in<S>(sdf, nullptr, iso, g);
     src.nseg = g.nseg;
+    src.recount_spins = (uint32_t)g_params.recount_spins;
     (void)ctl_bytes;
     SegBase *sb = (SegBase *)seg;
     uint16_t *info = reinterpret_cast<uint16_t *>(sb + g.nseg + 2);
@@ -2431,6 +2496,7 @@ int batch_count(diso_grid_t *grids, const BatchLayout &L, void *ws, void *ctl, h
     src.gflags = (uint32_t *)((uint8_t *)ctl + align256(ctl_bytes_for(Geo{})));
     src.ngrids = L.n;
     src.nseg = L.nseg;
+    src.recount_spins = (uint32_t)g_params.recount_spins;
     const int64_t nb = count_blocks(L.nseg, count_resident_per_cu<count_kernel<S, DISO_MC, true>>());
     hipLaunchKernelGGL((count_kernel<S, DISO_MC, true>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, src,
                        (const BatchGrid<S> *)P.desc, P.seg, P.info, (uint8_t *)ctl, nb);
@@ -2693,6 +2759,11 @@ int diso_set_param(int which, int value) {
                 return fail(DISO_E_INVALID, "count blocks per CU must be in [1, %d]", kCountBlocksPerCU);
             g_params.count_per_cu = value;
             return DISO_OK;
+        case DISO_PARAM_RECOUNT_SPINS:
+            if (value < 0 || value > (int)kSpinLimit)
+                return fail(DISO_E_INVALID, "recount spins must be in [0, %u]", kSpinLimit);
+            g_params.recount_spins = value;
+            return DISO_OK;
         default:
             return fail(DISO_E_INVALID, "unknown parameter %d", which);
     }
@@ -2702,6 +2773,7 @@ int diso_get_param(int which) {
     switch (which) {
         case DISO_PARAM_RUN: return g_params.run;
         case DISO_PARAM_COUNT_BLOCKS_PER_CU: return g_params.count_per_cu;
+        case DISO_PARAM_RECOUNT_SPINS: return g_params.recount_spins;
         default: return fail(DISO_E_INVALID, "unknown parameter %d", which);
     }
 }

@@ -196,8 +196,11 @@ int diso_batch_backward(int dtype, const diso_grid_t *grids, int n, void *seg, i
 /* Launch parameters, process-wide (set them before launching work; the autotuner in
  * diso_amd/autotune.py picks them per machine). Results do not depend on them.
  *  DISO_PARAM_RUN: segments per emission / backward block, 1..64 (default 32).
- *  DISO_PARAM_COUNT_BLOCKS_PER_CU: resident blocks per CU of the count pass, 1..4 (default 4). */
-enum { DISO_PARAM_RUN = 0, DISO_PARAM_COUNT_BLOCKS_PER_CU = 1 };
+ *  DISO_PARAM_COUNT_BLOCKS_PER_CU: resident blocks per CU of the count pass, 1..4 (default 4).
+ *  DISO_PARAM_RECOUNT_SPINS: reads of a predecessor's unpublished aggregate before a count block
+ *    counts that range itself, 0..2^20 (default 8192; 0 recounts every predecessor not already
+ *    published, which tests use to exercise that path). */
+enum { DISO_PARAM_RUN = 0, DISO_PARAM_COUNT_BLOCKS_PER_CU = 1, DISO_PARAM_RECOUNT_SPINS = 2 };
 int diso_set_param(int which, int value);
 int diso_get_param(int which);
 

@@ -160,5 +160,11 @@ def test_launch_parameters():
         assert autotune.current() == {"run": 16, "count_blocks_per_cu": 2}
         with pytest.raises(RuntimeError):
             autotune.apply({"run": 100, "count_blocks_per_cu": 2})
+        spins = L.diso_get_param(_lib.PARAM_RECOUNT_SPINS)
+        assert spins == 8192
+        assert L.diso_set_param(_lib.PARAM_RECOUNT_SPINS, -1) == -1
+        assert L.diso_set_param(_lib.PARAM_RECOUNT_SPINS, (1 << 20) + 1) == -1
+        assert L.diso_set_param(_lib.PARAM_RECOUNT_SPINS, 0) == 0 and L.diso_get_param(_lib.PARAM_RECOUNT_SPINS) == 0
+        assert L.diso_set_param(_lib.PARAM_RECOUNT_SPINS, spins) == 0
     finally:
         autotune.apply(before)

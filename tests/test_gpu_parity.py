@@ -241,11 +241,17 @@ def test_results_do_not_depend_on_launch_parameters():
 
     sdf_np, def_np = inputs((37, 50, 150), "noisy_box", 9, np.float32, True)
     before = autotune.current()
+    L = diso_amd._lib.lib()
+    spins = L.diso_get_param(diso_amd._lib.PARAM_RECOUNT_SPINS)
     outs = []
     try:
-        for run, cpc in ((1, 1), (7, 3), (32, 4), (64, 2)):
-            print(f"case launch run={run} count_blocks_per_cu={cpc}", flush=True)
+        # recount spins 0: every count block counts the ranges of the predecessors that have not
+        # published when it first looks (the path that keeps the scan live when launches share
+        # the device)
+        for run, cpc, rs in ((1, 1, spins), (7, 3, 0), (32, 4, spins), (64, 2, 0)):
+            print(f"case launch run={run} count_blocks_per_cu={cpc} recount_spins={rs}", flush=True)
             autotune.apply({"run": run, "count_blocks_per_cu": cpc})
+            assert L.diso_set_param(diso_amd._lib.PARAM_RECOUNT_SPINS, rs) == 0
             res = []
             for mod in (diso_amd.DiffMC(), diso_amd.DiffDMC()):
                 sdf = torch.tensor(sdf_np, device=DEV, requires_grad=True)
@@ -256,6 +262,7 @@ def test_results_do_not_depend_on_launch_parameters():
             outs.append(res)
     finally:
         autotune.apply(before)
+        L.diso_set_param(diso_amd._lib.PARAM_RECOUNT_SPINS, spins)
     for other in outs[1:]:
         for a, b in zip(outs[0], other):
             assert np.array_equal(a, b)
