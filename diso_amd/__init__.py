@@ -53,10 +53,14 @@ class _Extract(torch.autograd.Function):
         counted.quads = None
         ctx.save_for_backward(grid, deform, faces if ex.algo == _lib.DMC else None)
         ctx.mark_non_differentiable(faces)
+        # no zero-filled gradient for the (integer) faces, nor for unused vertices
+        ctx.set_materialize_grads(False)
         return verts, faces
 
     @staticmethod
     def backward(ctx, adj_verts, adj_faces):
+        if adj_verts is None:  # the vertices do not reach the loss: zero gradients
+            return None, None, None, None, None
         grid, deform, quads = ctx.saved_tensors
         ctx.counted.quads = quads
         grad_grid = torch.empty_like(grid)

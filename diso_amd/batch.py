@@ -96,6 +96,7 @@ class _BatchExtract(torch.autograd.Function):
         ctx.st = st
         ctx.save_for_backward(*sdfs, *[x if x is not None else None for x in deforms])
         ctx.mark_non_differentiable(*faces)
+        ctx.set_materialize_grads(False)  # no zero-filled gradients for faces or unused vertices
         return (*verts, *faces)
 
     @staticmethod

@@ -1294,18 +1294,26 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
         const int num = __popc(em);
         S sx = S(0), sy = S(0), sz = S(0);
         if (slots) {
-            while (em) {  // ascending edge order
-                const int e = __ffs(em) - 1;
-                em &= em - 1;
+            // the slots of all twelve edges first (independent reads), then the patch's edges'
+            // vertices, added in ascending edge order
+            const int nx = src + 1 < kWave ? src + 1 : src;
+            const int cap = nslot > 0 ? nslot - 1 : 0;
+            int sl[12];
+#pragma unroll
+            for (int e = 0; e < 12; ++e) {
                 const int v = (int)(kKinds >> (5 * e)) & 31, kd = v & 7, top = v >> 3;
                 // a top edge belongs to the next position (lane src + 1, or the halo after lane 63)
-                int slot = !top ? L.slot[kd][src] : (src == kWave - 1 ? L.hslot[(kd + 1) >> 1] : L.slot[kd][src + 1]);
+                const int sv = !top ? L.slot[kd][src] : (src == kWave - 1 ? L.hslot[(kd + 1) >> 1] : L.slot[kd][nx]);
                 // only a NaN sample can make a patch edge not cross: keep the read inside the table
-                slot = slot < nslot ? slot : (nslot > 0 ? nslot - 1 : 0);
-
-                sx += L.ev[0][slot];
-                sy += L.ev[1][slot];
-                sz += L.ev[2][slot];
+                sl[e] = sv < nslot ? sv : cap;
+            }
+#pragma unroll
+            for (int e = 0; e < 12; ++e) {
+                if ((em >> e) & 1) {
+                    sx += L.ev[0][sl[e]];
+                    sy += L.ev[1][sl[e]];
+                    sz += L.ev[2][sl[e]];
+                }
             }
         } else {
             const int z = z0 + src;
@@ -1341,7 +1349,7 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
 }
 
 template <typename S, bool DEF>
-__global__ __launch_bounds__(kBlock) void emit_dmc_kernel(In<S> in, const SegBase *__restrict__ seg,
+__global__ __launch_bounds__(kBlock, 6) void emit_dmc_kernel(In<S> in, const SegBase *__restrict__ seg,
                                                           S *__restrict__ verts, void *__restrict__ faces,
                                                           uint8_t *__restrict__ vinfo, OutCfg out, Norm<S> nm,
                                                           int run, int64_t nblocks) {
@@ -1814,7 +1822,7 @@ __global__ __launch_bounds__(kBlock, 7) void emit_mc_batch_kernel(const BatchGri
 }
 
 template <typename S, bool DEF>
-__global__ __launch_bounds__(kBlock) void emit_dmc_batch_kernel(const BatchGrid<S> *__restrict__ grids,
+__global__ __launch_bounds__(kBlock, 6) void emit_dmc_batch_kernel(const BatchGrid<S> *__restrict__ grids,
                                                                 const int32_t *__restrict__ gl,
                                                                 const int64_t *__restrict__ rf, int m,
                                                                 const SegBase *__restrict__ seg,

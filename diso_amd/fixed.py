@@ -64,6 +64,7 @@ class _FixedExtract(torch.autograd.Function):
         ctx.fx, ctx.iso, ctx.flags, ctx.dims, ctx.seg = fx, iso, flags, dims, seg
         ctx.save_for_backward(grid, deform, faces if dmc else None, vinfo if dmc else None)
         ctx.mark_non_differentiable(faces, counts)
+        ctx.set_materialize_grads(False)  # no zero-filled gradients for faces, counts or unused vertices
         return verts, faces, counts
 
     @staticmethod
