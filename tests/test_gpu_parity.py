@@ -317,3 +317,28 @@ def test_fp32_vertices_within_1e6_of_fp64_oracle(algo, normalize):
     assert np.array_equal(f.cpu().numpy(), rf)
     err = np.abs(v.cpu().numpy().astype(np.float64) - rv).max()
     assert err <= 1e-6 * np.abs(rv).max(), err
+
+
+@pytest.mark.parametrize("algo", [O.MC, O.DMC])
+def test_backward_makes_no_face_gradient(algo):
+    """The faces are integer outputs: the backward allocates the two input gradients (and the DMC
+    edge scratch) but no zero-filled gradient of the faces."""
+    print(f"case no face gradient algo={algo}", flush=True)
+    sdf_np, def_np = inputs((80, 80, 80), "noise", 4, np.float32, True)
+    sdf = torch.tensor(sdf_np, device=DEV, requires_grad=True)
+    deform = torch.tensor(def_np, device=DEV, requires_grad=True)
+    kw = {"return_quads": True} if algo == O.DMC else {}
+    mod = _module(algo, np.float32)
+    v, f = mod(sdf, deform, **kw)
+    up = torch.randn(v.shape, device=DEV, generator=torch.Generator(device=DEV).manual_seed(1))
+    face_bytes = f.numel() * f.element_size()
+    scratch = 0 if algo == O.MC else 12 * f.shape[0]
+    assert face_bytes > 2 << 20  # large enough to show above the allocator's rounding
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
+    torch.autograd.backward(v, grad_tensors=up)
+    torch.cuda.synchronize()
+    peak = torch.cuda.max_memory_allocated() - base
+    grads = (sdf.numel() + deform.numel()) * 4
+    assert peak < grads + scratch + face_bytes // 2, (peak, grads, scratch, face_bytes)
