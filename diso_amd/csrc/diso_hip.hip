@@ -198,13 +198,18 @@ __device__ __forceinline__ int popc3(int m) { return (m & 1) + ((m >> 1) & 1) + 
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// Set bits of a wave-wide mask below the calling lane (v_mbcnt_lo / v_mbcnt_hi: two VALU
+// instructions, where masking with a per-lane 64-bit shift and counting takes six).
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 // Exclusive prefix over the wave of a per-lane value in [0, 7] (three ballots); *total gets the
 // wave sum. Must be reached by all 64 lanes.
 __device__ __forceinline__ int wave_excl_small(int v, int *total) {
-    uint64_t lt = (1ull << lane_id()) - 1ull;
     uint64_t b0 = __ballot(v & 1), b1 = __ballot(v & 2), b2 = __ballot(v & 4);
     *total = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
-    return __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt);
+    return lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2);
 }
 
 __device__ __forceinline__ int64_t wave_sum64(int64_t v) {
@@ -940,7 +945,8 @@ struct McSegIn {
     int64_t s;
     SegPos sp;
     uint32_t w00, w10, w01, w11, n00, n10;
-    int64_t vabs, fabs, v10, v01, v11;
+    int64_t vabs, fabs;
+    int32_t d10, d01, d11;  // first vertices of rows +x, +y, +x+y relative to the segment's
 };
 
 template <typename S>
@@ -957,9 +963,9 @@ __device__ __forceinline__ McSegIn mc_load(const In<S> &in, const SegCtx &sc, in
                   s11 = seg_index(g, px + 1, py + 1, chunk);
     d.vabs = seg[s].v;
     d.fabs = seg[s].f;
-    d.v10 = seg[s10].v;
-    d.v01 = seg[s01].v;
-    d.v11 = seg[s11].v;
+    d.d10 = (int32_t)(seg[s10].v - d.vabs);
+    d.d01 = (int32_t)(seg[s01].v - d.vabs);
+    d.d11 = (int32_t)(seg[s11].v - d.vabs);
     d.w00 = info[s * kWave + lane];
     d.w10 = info[s10 * kWave + lane];
     d.w01 = info[s01 * kWave + lane];
@@ -990,9 +996,9 @@ __device__ __forceinline__ void emit_mc_segment(const In<S> &in, const SegCtx &s
     int tv, tt, t;
     const int p00 = wave_excl_small(popc3(m00), &tv);
     const int pt = wave_excl_small(ntri, &tt);
-    const int32_t r10 = (int32_t)(d.v10 - vabs) + wave_excl_small(popc3(m10), &t);
-    const int32_t r01 = (int32_t)(d.v01 - vabs) + wave_excl_small(popc3(m01), &t);
-    const int32_t r11 = (int32_t)(d.v11 - vabs) + wave_excl_small(popc3(m11), &t);
+    const int32_t r10 = d.d10 + wave_excl_small(popc3(m10), &t);
+    const int32_t r01 = d.d01 + wave_excl_small(popc3(m01), &t);
+    const int32_t r11 = d.d11 + wave_excl_small(popc3(m11), &t);
 
     {
         int r = p00;
@@ -1163,7 +1169,8 @@ struct DmcSegIn {
     SegPos sp;
     uint32_t w0, w10, w01, w11, h00, h10, h01;
     uint32_t cur[4], prev[4];  // codes of cells (-i,-j,0) and (-i,-j,-1), index 2i + j
-    int64_t vabs, pabs, rowf[4];
+    int64_t vabs, pabs;
+    int32_t rowd[4];  // first DMC vertex of row (-i, -j), relative to the segment's
 };
 
 template <typename S>
@@ -1195,7 +1202,7 @@ __device__ __forceinline__ DmcSegIn dmc_load(const In<S> &in, const SegCtx &sc, 
         for (int j = 0; j < 2; ++j) {
             const bool row = px - i >= 0 && py - j >= 0;  // wave-uniform
             const int64_t r = row ? seg_index(g, px - i, py - j, chunk) : s;
-            d.rowf[2 * i + j] = row ? seg[r].f : d.pabs;
+            d.rowd[2 * i + j] = row ? (int32_t)(seg[r].f - d.pabs) : 0;
             d.cur[2 * i + j] = (i || j) ? (row ? info[r * kWave + lane] : 0u) : d.w0;
             d.prev[2 * i + j] = (row && !(i && j) && pz >= 1) ? info[r * kWave + lane - 1] : 0u;
         }
@@ -1218,6 +1225,10 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
     // Crossing masks of the rows the cells' edges start from: 10, 01, 11 (this z) and the halo.
     const int m00 = info_mask(w0), m10 = info_mask(d.w10), m01 = info_mask(d.w01), m11 = info_mask(d.w11);
     const int h00 = info_mask(d.h00), h10 = info_mask(d.h10), h01 = info_mask(d.h01);
+    // prefix of the segment's own patches; also the first vertices of row (0, 0)'s cells below
+    const int np = info_count(w0);
+    int tq, tp;
+    const int pp = wave_excl_small(np, &tp);
     // Codes and first vertices of the cells (-i, -j, 0) and (-i, -j, -1).
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -1225,17 +1236,14 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
         for (int j = 0; j < 2; ++j) {
             const uint32_t cur = d.cur[2 * i + j], prev = d.prev[2 * i + j];
             int t;
-            const int b = wave_excl_small(info_count(cur), &t);
-            if (lane == 0) L.rowb[2 * i + j] = (int32_t)(d.rowf[2 * i + j] - pabs);
+            const int b = (i || j) ? wave_excl_small(info_count(cur), &t) : pp;  // cur of (0, 0) is w0
+            if (lane == 0) L.rowb[2 * i + j] = d.rowd[2 * i + j];
             L.code[4 * i + 2 * j][lane] = (uint8_t)info_code(cur);
             L.pre[4 * i + 2 * j][lane] = (int16_t)b;
             L.code[4 * i + 2 * j + 1][lane] = (uint8_t)info_code(prev);
             L.pre[4 * i + 2 * j + 1][lane] = (int16_t)(b - info_count(prev));
         }
-    const int np = info_count(w0);
-    int tq, tp;
     const int pq = wave_excl_small(popc3(m00), &tq);
-    const int pp = wave_excl_small(np, &tp);
     for (int k = 0; k < np; ++k) L.patch[pp + k] = (uint8_t)(lane * 4 + k);
     {
         // an owned crossing edge leaves the inside when the position's sample is >= iso
@@ -1250,13 +1258,12 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
     const int kbits = (m00 & 7) | (((m10 >> 1) & 3) << 3) | ((m01 & 1) << 5) | (((m01 >> 2) & 1) << 6) |
                       (((m11 >> 2) & 1) << 7);
     const int hbits = (h00 & 3) | (((h10 >> 1) & 1) << 2) | ((h01 & 1) << 3);
-    const uint64_t lt = (1ull << lane) - 1ull;
     int nslot = 0, myslot[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
         const bool on = k < 8 ? ((kbits >> k) & 1) : ((hbits >> (k - 8)) & 1);
         const uint64_t bal = __ballot(on);
-        const int ex = __popcll(bal & lt);
+        const int ex = lanes_below(bal);
         myslot[k] = on ? nslot + ex : -1;
         if (k < 8)
             L.slot[k][lane] = (uint8_t)myslot[k];
@@ -1526,7 +1533,8 @@ __device__ __forceinline__ bool seg_touched(const SegBase *__restrict__ seg, con
 struct BwdSegIn {
     int64_t s;
     SegPos sp;
-    int64_t v, vm0, v0m;
+    int64_t v;
+    int32_t dm0, d0m;  // first vertices of rows -x, -y relative to the segment's
     uint32_t w00, wm0, w0m, wzm;
 };
 
@@ -1539,8 +1547,8 @@ __device__ __forceinline__ BwdSegIn bwd_load(const Geo &g, const SegCtx &sc, int
     const int64_t sm0 = sp.px > 0 ? seg_index(g, sp.px - 1, sp.py, sp.chunk) : -1;
     const int64_t s0m = sp.py > 0 ? seg_index(g, sp.px, sp.py - 1, sp.chunk) : -1;
     d.v = seg[s].v;
-    d.vm0 = sm0 >= 0 ? seg[sm0].v : d.v;
-    d.v0m = s0m >= 0 ? seg[s0m].v : d.v;
+    d.dm0 = sm0 >= 0 ? (int32_t)(seg[sm0].v - d.v) : 0;
+    d.d0m = s0m >= 0 ? (int32_t)(seg[s0m].v - d.v) : 0;
     d.w00 = d.wm0 = d.w0m = d.wzm = 0;
     if constexpr (!RECOMP) {
         const uint16_t *info = sc.info;
@@ -1590,8 +1598,8 @@ __device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, c
     const int mask = in_data(in, px, py, pz) ? (m00 | (inc_m << 3)) : 0;
     int t, total;
     const int32_t r00 = wave_excl_small(popc3(m00), &t);
-    const int32_t rm0 = (int32_t)(sg.vm0 - vabs) + wave_excl_small(popc3(mm0), &t);
-    const int32_t r0m = (int32_t)(sg.v0m - vabs) + wave_excl_small(popc3(m0m), &t);
+    const int32_t rm0 = sg.dm0 + wave_excl_small(popc3(mm0), &t);
+    const int32_t r0m = sg.d0m + wave_excl_small(popc3(m0m), &t);
     const int first = wave_excl_small(__popc(mask), &total);
 
     L.id[0][lane] = r00;
@@ -2020,7 +2028,6 @@ __global__ __launch_bounds__(kSplitBlock) void split_emit_kernel(const void *__r
         run += s_pre[w];
         n1 += s_tot[w];
     }
-    const uint64_t lt = (1ull << lane) - 1ull;
     for (int64_t base = lo; base < hi; base += kSplitBlock) {
         const int64_t q = base + tid;
         const bool f = q < hi && first[q];
@@ -2028,7 +2035,7 @@ __global__ __launch_bounds__(kSplitBlock) void split_emit_kernel(const void *__r
         __syncthreads();  // s_w reuse
         if (lane == 0) s_w[wave] = (uint32_t)__popcll(bal);
         __syncthreads();
-        int64_t r1 = run + __popcll(bal & lt);
+        int64_t r1 = run + lanes_below(bal);
         uint32_t blk = 0;
 #pragma unroll
         for (int w = 0; w < kSplitWaves; ++w) {
