@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--learn-steps", type=int, default=100)
     p.add_argument("--serve-requests", type=int, default=20000, help="requests of the serving stream (0: skip it)")
     p.add_argument("--serve-load", type=float, default=0.95, help="arrival rate / the plain path's service rate")
-    p.add_argument("--serve-workers", type=int, default=2)
+    p.add_argument("--serve-workers", type=int, default=4)
     return p.parse_args()
 
 

@@ -73,7 +73,7 @@ class Request:
 
 
 class Server:
-    def __init__(self, dtype=torch.float32, device="cuda:0", budget_bytes=200 << 30, max_batch=64, workers=2,
+    def __init__(self, dtype=torch.float32, device="cuda:0", budget_bytes=200 << 30, max_batch=64, workers=4,
                  reference_compat=False, max_batch_voxels=1 << 20):
         self.dtype = dtype
         self.device = torch.device(device)
