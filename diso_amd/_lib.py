@@ -34,6 +34,7 @@ TOTAL_HAS_BELOW = 1 << 0
 TOTAL_HAS_ABOVE = 1 << 1
 TOTAL_HAS_NAN = 1 << 2
 TOTAL_SCAN_TIMEOUT = 1 << 3
+E_TIMEOUT = -4  # DISO_E_TIMEOUT: the control block must be zero-filled before its next use
 PARAM_RUN = 0
 PARAM_COUNT_BLOCKS_PER_CU = 1
 PARAM_RECOUNT_SPINS = 2

@@ -65,6 +65,8 @@ class _BatchExtract(torch.autograd.Function):
             ctl = st.ex.ctl_for(dev, n)
             rc = L.diso_batch_count(code, ctypes.addressof(d), n, _ptr(st.seg), nbytes, _ptr(ctl), ctl.numel(),
                                     stream)
+            if rc == _lib.E_TIMEOUT:
+                ctl.zero_()
             _lib.check(rc, "batch count pass")
         nbytes = st.seg.numel()
         both = _lib.TOTAL_HAS_BELOW | _lib.TOTAL_HAS_ABOVE
@@ -231,6 +233,8 @@ class BatchExtractor:
             ctl = self.ctl_for(dev, st.n)
             rc = L.diso_batch_count(self.code, ctypes.addressof(d), st.n, _ptr(st.seg), nbytes, _ptr(ctl),
                                     ctl.numel(), _stream(dev))
+            if rc == _lib.E_TIMEOUT:
+                ctl.zero_()
             _lib.check(rc, "batch count pass")
         sizes = [(int(d[i].totals.n_verts), int(d[i].totals.n_faces), int(d[i].totals.flags)) for i in range(st.n)]
         return _Counted(st, sizes, [(g.data_ptr(), int(d[i].algo), tuple(g.shape)) for i, g in enumerate(sdfs)])

@@ -89,6 +89,13 @@ class Extractor:
             self._ctl[key] = ctl
         return ctl
 
+    def reset_ctl(self):
+        """Zero-fills the control blocks (queued on the current streams). Needed only after a scan
+        timeout: a block that started after its call had given up on it may have left a word in
+        the buffer the next call expects to be clean."""
+        for ctl in self._ctl.values():
+            ctl.zero_()
+
     def count(self, grid, iso, flags):
         dims = tuple(int(d) for d in grid.shape)
         L = _lib.lib()
@@ -100,6 +107,8 @@ class Extractor:
         tot = _lib.Totals()
         rc = L.diso_count(self.algo, self.code, _ptr(grid), *dims, float(iso), flags, _ptr(seg), nseg,
                           _ptr(ctl), ctl.numel(), ctypes.byref(tot), _stream(grid.device))
+        if rc == _lib.E_TIMEOUT:
+            self.reset_ctl()
         _lib.check(rc, "count pass")
         return Counted(seg, tot, dims, float(iso), flags)
 
@@ -155,6 +164,7 @@ class Extractor:
         tot.n_verts, tot.n_faces = (other, crossing) if dmc else (crossing, other)
         tot.flags, tot.n_crossing = tflags, crossing
         if tflags & _lib.TOTAL_SCAN_TIMEOUT:
+            self.reset_ctl()
             raise RuntimeError("diso_amd: count pass failed: single-pass scan did not complete")
         counted = Counted(seg, tot, dims, float(iso), flags)
         if tot.n_verts <= vcap and tot.n_faces <= fcap:

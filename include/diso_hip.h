@@ -53,7 +53,8 @@ enum {
     DISO_E_INVALID = -1,  /* bad argument                                                   */
     DISO_E_HIP = -2,      /* HIP runtime error                                              */
     DISO_E_CAPACITY = -3, /* a workspace or output buffer is too small                      */
-    DISO_E_TIMEOUT = -4   /* the single-pass scan did not complete (never expected)         */
+    DISO_E_TIMEOUT = -4   /* the single-pass scan did not complete (never expected); zero-fill
+                             the control block before its next use                          */
 };
 
 /* Status bits of diso_totals_t.flags. */

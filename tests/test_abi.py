@@ -168,3 +168,16 @@ def test_launch_parameters():
         assert L.diso_set_param(_lib.PARAM_RECOUNT_SPINS, spins) == 0
     finally:
         autotune.apply(before)
+
+
+def test_control_blocks_are_cleared_after_a_scan_timeout():
+    """After a scan timeout the extractor zero-fills its control blocks (a late block may have left
+    a word in the buffer the next call expects clean); the plain count path does it when the
+    library reports DISO_E_TIMEOUT."""
+    from diso_amd.engine import Extractor
+
+    ex = Extractor(_lib.MC, torch.float32)
+    ex._ctl = {("cpu", 1): torch.full((64,), 7, dtype=torch.uint8), ("cpu", 2): torch.ones(16, dtype=torch.uint8)}
+    ex.reset_ctl()
+    assert all(int(c.sum()) == 0 for c in ex._ctl.values())
+    assert _lib.E_TIMEOUT == -4
