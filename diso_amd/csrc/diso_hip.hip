@@ -1509,7 +1509,7 @@ template <typename S>
 struct BwdLds {
     int32_t id[6][kWave];
     uint16_t item[kWave * 6];
-    Grad<S> res[kWave];
+    Grad<S> res[2 * kWave];
 };
 
 // Whether segment t (-1: none) owns a crossing edge.
@@ -1617,21 +1617,30 @@ __device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, c
     wave_lds_sync();
 
     const int last = first + __popc(mask);
-    for (int b = 0; b < total; b += kWave) {
-        const int k = b + lane;
-        if (k < total) {
-            const int it = L.item[k], src = it >> 3, slot = it & 7;
-            const int axis = slot % 3, end = slot >= 3;
-            // the edge runs from (x, y, z) along `axis`; the vertex is its start (slots 0-2) or its end
-            const int x = px - (slot == 3), y = py - (slot == 4), z = z0 + src - (slot == 5);
-            const int ex = x + (axis == 0), ey = y + (axis == 1), ez = z + (axis == 2);
-            const S d0 = sample(in, x, y, z), d1 = sample(in, ex, ey, ez);
-            const V3<S> f0 = deform_at<S, DEF>(in, x, y, z), f1 = deform_at<S, DEF>(in, ex, ey, ez);
-            L.res[lane] = edge_adjoint<S, DEF>(x, y, z, axis, end, d0, d1, f0, f1, iso,
-                                               load_adj(adj, vbase + L.id[slot][src]));
+    // Adjoint of item k (an edge and the endpoint that is this segment's vertex).
+    auto item = [&](int k) {
+        const int it = L.item[k], src = it >> 3, slot = it & 7;
+        const int axis = slot % 3, end = slot >= 3;
+        // the edge runs from (x, y, z) along `axis`; the vertex is its start (slots 0-2) or its end
+        const int x = px - (slot == 3), y = py - (slot == 4), z = z0 + src - (slot == 5);
+        const int ex = x + (axis == 0), ey = y + (axis == 1), ez = z + (axis == 2);
+        const S d0 = sample(in, x, y, z), d1 = sample(in, ex, ey, ez);
+        const V3<S> f0 = deform_at<S, DEF>(in, x, y, z), f1 = deform_at<S, DEF>(in, ex, ey, ez);
+        return edge_adjoint<S, DEF>(x, y, z, axis, end, d0, d1, f0, f1, iso, load_adj(adj, vbase + L.id[slot][src]));
+    };
+    // Rounds of up to 128 items: with more than 64 left, each lane takes two (the second clamped
+    // to the last item, its result unused), so both gathers are in flight together.
+    for (int b = 0; b < total; b += 2 * kWave) {
+        if (b + kWave < total) {  // wave-uniform
+            const int k1 = b + kWave + lane < total ? b + kWave + lane : total - 1;
+            const Grad<S> r0 = item(b + lane), r1 = item(k1);
+            L.res[lane] = r0;
+            L.res[kWave + lane] = r1;
+        } else if (b + lane < total) {
+            L.res[lane] = item(b + lane);
         }
         wave_lds_sync();
-        const int lo = first > b ? first : b, hi = last < b + kWave ? last : b + kWave;
+        const int lo = first > b ? first : b, hi = last < b + 2 * kWave ? last : b + 2 * kWave;
         for (int j = lo; j < hi; ++j) {  // this lane's items of the round, in slot order
             const Grad<S> r = L.res[j - b];
             acc.g += r.g;
