@@ -14,6 +14,10 @@ reports the slowest one against HBM peak using its algorithmic bytes (sdf / defo
 samples it must touch plus the vertices, faces and dense gradients it must write).
 `cpu_baseline` runs the OpenMP CPU oracle (the reference's per-cell algorithm) on the same
 workload on the host cores.
+
+Further legs, keys of the same line: `batched` (64 grids, N in 32..256) and `batched_small` (512
+grids, N in 16..64), each as one batched call per pass against a loop of per-grid calls;
+`serving` (continuous batching under Poisson arrivals); `learn` (graph-captured 512^3 fit).
 """
 import argparse
 import ctypes
@@ -43,6 +47,8 @@ def parse():
     p.add_argument("--cpu-reps", type=int, default=3)
     p.add_argument("--copy-mib", type=int, default=256)
     p.add_argument("--batch-count", type=int, default=64, help="grids of the uneven batch (0: skip it)")
+    p.add_argument("--batch-small-count", type=int, default=512,
+                   help="grids of the small-grid batch, N in 16..64 (0: skip it)")
     p.add_argument("--batch-reps", type=int, default=5)
     p.add_argument("--learn-n", type=int, default=512, help="learnable-sdf fit grid (0: skip the fit)")
     p.add_argument("--learn-stored-n", type=int, default=384)
@@ -171,8 +177,8 @@ def profiled_traffic(phase, n):
     return None
 
 
-def batched_vs_plain(dev, count, seed, reps):
-    """The uneven batch: `count` grids, N uniform over {32,...,256}, mixed families, deform,
+def batched_vs_plain(dev, count, seed, reps, sizes=(32, 48, 64, 96, 128, 192, 256)):
+    """The uneven batch: `count` grids, N uniform over `sizes`, mixed families, deform,
     MC/DMC alternating, DMC returning quads; forward + backward of all grids per pass, timed as
     one BatchExtractor call and as a Python loop of per-grid DiffMC / DiffDMC calls."""
     import torch
@@ -180,7 +186,7 @@ def batched_vs_plain(dev, count, seed, reps):
     import diso_amd
     from diso_amd import workloads
 
-    data = workloads.uneven_batch_torch(count, seed=seed, device=dev)
+    data = workloads.uneven_batch_torch(count, seed=seed, sizes=sizes, device=dev)
     voxels = sum(n ** 3 for *_, n in data)
     algos = [a for _, _, a, _, _ in data]
     mods = {"mc": diso_amd.DiffMC(torch.float32), "dmc": diso_amd.DiffDMC(torch.float32)}
@@ -222,8 +228,8 @@ def batched_vs_plain(dev, count, seed, reps):
         torch.cuda.synchronize()
         res[name] = (time.perf_counter() - t0) / reps
     return {
-        "workload": f"{count} grids, N uniform over 32..256 (total {voxels} voxels), sphere / rounded box / "
-                    "blobs / noise, deform U(-0.3,0.3), MC and DMC alternating, fwd+bwd",
+        "workload": f"{count} grids, N uniform over {{{','.join(str(x) for x in sizes)}}} (total {voxels} voxels), "
+                    "sphere / rounded box / blobs / noise, deform U(-0.3,0.3), MC and DMC alternating, fwd+bwd",
         "voxels": int(voxels),
         "batched_gvoxel_s": round(voxels / res["batched"] / 1e9, 3),
         "plain_loop_gvoxel_s": round(voxels / res["plain_loop"] / 1e9, 3),
@@ -497,6 +503,10 @@ def main():
         del sets, mods
         torch.cuda.empty_cache()
         out["batched"] = batched_vs_plain(dev, args.batch_count, args.seed, args.batch_reps)
+    if args.batch_small_count > 0:
+        torch.cuda.empty_cache()
+        out["batched_small"] = batched_vs_plain(dev, args.batch_small_count, args.seed + 1, args.batch_reps,
+                                                sizes=(16, 24, 32, 48, 64))
     if args.serve_requests > 0:
         torch.cuda.empty_cache()
         out["serving"] = serving(dev, args.serve_requests, args.serve_load, args.seed, args.serve_workers)
