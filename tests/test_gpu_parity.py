@@ -259,6 +259,10 @@ def test_results_do_not_depend_on_launch_parameters():
                 v, f = mod(sdf, deform)
                 (v * v).sum().backward()
                 res += [v.detach().cpu().numpy(), f.cpu().numpy(), sdf.grad.cpu().numpy(), deform.grad.cpu().numpy()]
+            # the batched count pass (one scan over the segments of several grids)
+            grids = [torch.tensor(sdf_np, device=DEV), torch.tensor(sdf_np[:20, :33, :70].copy(), device=DEV)]
+            for v, f in diso_amd.BatchExtractor()(grids, ["mc", "dmc"], return_quads=True):
+                res += [v.cpu().numpy(), f.cpu().numpy()]
             outs.append(res)
     finally:
         autotune.apply(before)
