@@ -12,6 +12,7 @@ kernels run, only the launches are shared. A batch costs one host synchronisatio
 """
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -34,6 +35,33 @@ class GridT(ctypes.Structure):
 _ALGOS = {"mc": _lib.MC, "dmc": _lib.DMC, _lib.MC: _lib.MC, _lib.DMC: _lib.DMC}
 
 
+def _grid_np_dtype():
+    """A numpy record type laid out exactly as GridT (totals flattened to t_*), so that the fields
+    of all descriptors are read and written as columns instead of one ctypes access each."""
+    conv = {ctypes.c_int32: np.int32, ctypes.c_int64: np.int64, ctypes.c_double: np.float64, ctypes.c_void_p: np.uint64}
+    names, formats, offsets = [], [], []
+    for name, ct in GridT._fields_:
+        off = getattr(GridT, name).offset
+        if ct is _lib.Totals:
+            for tn, tct in _lib.Totals._fields_:
+                names.append("t_" + tn)
+                formats.append(conv[tct])
+                offsets.append(off + getattr(_lib.Totals, tn).offset)
+        else:
+            names.append(name)
+            formats.append(conv[ct])
+            offsets.append(off)
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets, "itemsize": ctypes.sizeof(GridT)})
+
+
+_GRID_NP = _grid_np_dtype()
+
+
+def _table(desc):
+    """Column view of a GridT array: writes go to the descriptors themselves."""
+    return np.frombuffer(desc, dtype=_GRID_NP)
+
+
 class _State:
     """Per-call state shared by forward and backward: descriptors, workspace, saved outputs."""
 
@@ -41,6 +69,7 @@ class _State:
         self.ex = ex
         self.n = n
         self.desc = (GridT * n)()
+        self.table = _table(self.desc)
         self.seg = None
         self.quads = [None] * n
         self.vinfo = [None] * n
@@ -70,29 +99,37 @@ class _BatchExtract(torch.autograd.Function):
             _lib.check(rc, "batch count pass")
         nbytes = st.seg.numel()
         both = _lib.TOTAL_HAS_BELOW | _lib.TOTAL_HAS_ABOVE
+        a = st.table
+        fl = a["t_flags"]
+        nan = np.flatnonzero(fl & _lib.TOTAL_HAS_NAN)
+        if nan.size:
+            raise RuntimeError(f"diso_amd: sdf grid {int(nan[0])} of the batch contains NaN values")
+        empty = (fl & both) != both
+        # diso/__init__.py:49-50,103-104: a grid's early-out
+        a["t_n_verts"][empty] = 0
+        a["t_n_faces"][empty] = 0
+        st.empty = empty.tolist()
+        is_dmc = (a["algo"] == _lib.DMC).tolist()
+        nvs, nfs = a["t_n_verts"].tolist(), a["t_n_faces"].tolist()
         verts, faces = [], []
+        vp, fp, vip = [0] * n, [0] * n, [0] * n
         for i in range(n):
-            t = d[i].totals
-            if t.flags & _lib.TOTAL_HAS_NAN:
-                raise RuntimeError(f"diso_amd: sdf grid {i} of the batch contains NaN values")
-            dmc = d[i].algo == _lib.DMC
-            st.empty[i] = (t.flags & both) != both
+            dmc = is_dmc[i]
             if st.empty[i]:
-                # diso/__init__.py:49-50,103-104: the grid's early-out
-                d[i].totals.n_verts = 0
-                d[i].totals.n_faces = 0
                 verts.append(torch.zeros((0, 3), dtype=st.ex.dtype, device=dev))
                 faces.append(torch.zeros((0, 4 if dmc else 3), dtype=torch.int32, device=dev))
                 continue
-            v = torch.empty((t.n_verts, 3), dtype=st.ex.dtype, device=dev)
-            f = torch.empty((t.n_faces, 4 if dmc else 3), dtype=torch.int64, device=dev)
-            vi = torch.empty(t.n_verts if dmc else 0, dtype=torch.uint8, device=dev)
-            d[i].verts, d[i].faces, d[i].vinfo = v.data_ptr(), f.data_ptr(), vi.data_ptr() if dmc else None
-            d[i].vert_capacity, d[i].face_capacity = t.n_verts, t.n_faces
+            v = torch.empty((nvs[i], 3), dtype=st.ex.dtype, device=dev)
+            f = torch.empty((nfs[i], 4 if dmc else 3), dtype=torch.int64, device=dev)
+            vp[i], fp[i] = v.data_ptr(), f.data_ptr()
             if dmc:
+                vi = torch.empty(nvs[i], dtype=torch.uint8, device=dev)
+                vip[i] = vi.data_ptr()
                 st.quads[i], st.vinfo[i] = f, vi
             verts.append(v)
             faces.append(f)
+        a["verts"], a["faces"], a["vinfo"] = vp, fp, vip
+        a["vert_capacity"], a["face_capacity"] = a["t_n_verts"], a["t_n_faces"]
         rc = L.diso_batch_emit(code, ctypes.addressof(d), n, _ptr(st.seg), nbytes, stream)
         _lib.check(rc, "batch emit pass")
         ctx.st = st
@@ -111,9 +148,9 @@ class _BatchExtract(torch.autograd.Function):
         d = st.desc
         g_sdf, g_def = [None] * n, [None] * n
         keep = []
+        ap, sp, dp = [0] * n, [0] * n, [0] * n
         for i in range(n):
             gv = grads[i]
-            d[i].adj_verts = d[i].adj_sdf = d[i].adj_deform = None
             if gv is None or st.empty[i]:
                 continue
             gv = gv.contiguous()
@@ -122,10 +159,11 @@ class _BatchExtract(torch.autograd.Function):
             g_sdf[i] = torch.empty_like(sdfs[i])
             if deforms[i] is not None:
                 g_def[i] = torch.empty_like(deforms[i])
+                dp[i] = g_def[i].data_ptr()
             keep.append(gv)
-            d[i].adj_verts = gv.data_ptr()
-            d[i].adj_sdf = g_sdf[i].data_ptr()
-            d[i].adj_deform = None if g_def[i] is None else g_def[i].data_ptr()
+            ap[i], sp[i] = gv.data_ptr(), g_sdf[i].data_ptr()
+        a = st.table
+        a["adj_verts"], a["adj_sdf"], a["adj_deform"] = ap, sp, dp
         if keep:
             dev = sdfs[0].device
             nsc = L.diso_batch_scratch_bytes(st.ex.code, ctypes.addressof(d), n)
@@ -182,6 +220,12 @@ class BatchExtractor:
         dev = grids[0].device
         st = _State(self, n)
         sdfs, defs = [], []
+        base = _lib.FLAG_VIRTUAL_PAD | _lib.FLAG_SHIFT | _lib.FLAG_INDEX64
+        if normalize:
+            base |= _lib.FLAG_NORMALIZE
+        if self.recompute_backward:
+            base |= _lib.FLAG_RECOMPUTE
+        col = {k: [0] * n for k in ("algo", "flags", "X", "Y", "Z", "sdf", "deform")}
         for i in range(n):
             g, df = grids[i], deforms[i]
             if not isinstance(g, torch.Tensor) or g.dim() != 3:
@@ -200,19 +244,15 @@ class BatchExtractor:
             df = None if df is None else df.contiguous()
             sdfs.append(g)
             defs.append(df)
-            flags = _lib.FLAG_VIRTUAL_PAD | _lib.FLAG_SHIFT | _lib.FLAG_INDEX64
-            if normalize:
-                flags |= _lib.FLAG_NORMALIZE
-            if self.reference_compat and algo == _lib.DMC:
-                flags |= _lib.FLAG_REFERENCE_COMPAT
-            if self.recompute_backward:
-                flags |= _lib.FLAG_RECOMPUTE
-            dsc = st.desc[i]
-            dsc.algo, dsc.flags = algo, flags
-            dsc.X, dsc.Y, dsc.Z = (int(x) for x in g.shape)
-            dsc.iso = float(isovalue)
-            dsc.sdf = g.data_ptr()
-            dsc.deform = None if df is None else df.data_ptr()
+            col["algo"][i] = algo
+            col["flags"][i] = base | (_lib.FLAG_REFERENCE_COMPAT if self.reference_compat and algo == _lib.DMC else 0)
+            col["X"][i], col["Y"][i], col["Z"][i] = g.shape
+            col["sdf"][i] = g.data_ptr()
+            col["deform"][i] = 0 if df is None else df.data_ptr()
+        a = st.table
+        for k, v in col.items():
+            a[k] = v
+        a["iso"] = float(isovalue)
         return st, sdfs, defs, dev
 
     def count(self, grids, algos, isovalue=0.0, normalize=True):
@@ -236,8 +276,10 @@ class BatchExtractor:
             if rc == _lib.E_TIMEOUT:
                 ctl.zero_()
             _lib.check(rc, "batch count pass")
-        sizes = [(int(d[i].totals.n_verts), int(d[i].totals.n_faces), int(d[i].totals.flags)) for i in range(st.n)]
-        return _Counted(st, sizes, [(g.data_ptr(), int(d[i].algo), tuple(g.shape)) for i, g in enumerate(sdfs)])
+        a = st.table
+        sizes = list(zip(a["t_n_verts"].tolist(), a["t_n_faces"].tolist(), a["t_flags"].tolist()))
+        algo = a["algo"].tolist()
+        return _Counted(st, sizes, [(g.data_ptr(), algo[i], tuple(g.shape)) for i, g in enumerate(sdfs)])
 
     def __call__(self, grids, algos, deforms=None, isovalue=0.0, normalize=True, return_quads=False, counted=None):
         n = len(grids)
@@ -246,20 +288,21 @@ class BatchExtractor:
         st, sdfs, defs, dev = self._describe(grids, algos, deforms, isovalue, normalize)
         if counted is not None:
             # reuse the counting pass: same grids (storage, dims, algorithm) and settings
+            a, algo = st.table, st.table["algo"].tolist()
             same = counted.st is not None and counted.st.n == n and all(
-                (g.data_ptr(), int(st.desc[i].algo), tuple(g.shape)) == counted.key[i] and
-                counted.st.desc[i].flags == st.desc[i].flags and counted.st.desc[i].iso == st.desc[i].iso
-                for i, g in enumerate(sdfs))
+                (g.data_ptr(), algo[i], tuple(g.shape)) == counted.key[i] for i, g in enumerate(sdfs)) and \
+                np.array_equal(counted.st.table["flags"], a["flags"]) and np.array_equal(counted.st.table["iso"], a["iso"])
             if not same:
                 raise ValueError("counted: the handle's counting pass was not run on these grids and settings")
-            for i in range(n):
-                counted.st.desc[i].deform = st.desc[i].deform
+            counted.st.table["deform"] = a["deform"]
             st = counted.st
             counted.st = None  # a handle serves one call
         with torch.cuda.device(dev):
             outs = _BatchExtract.apply(st, *sdfs, *defs)
             verts, faces = list(outs[:n]), list(outs[n:])
-            for i in range(n):
-                if st.desc[i].algo == _lib.DMC and not return_quads and not st.empty[i]:
-                    faces[i] = split_quads(verts[i], faces[i])
+            if not return_quads:
+                algo = st.table["algo"].tolist()
+                for i in range(n):
+                    if algo[i] == _lib.DMC and not st.empty[i]:
+                        faces[i] = split_quads(verts[i], faces[i])
         return list(zip(verts, faces))
