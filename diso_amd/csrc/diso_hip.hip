@@ -1734,7 +1734,9 @@ __global__ __launch_bounds__(kBlock) void dmc_edge_grad_kernel(const void *__res
     __syncthreads();
     // nq may be a capacity: the quads are the first tot->v rows (the count pass's total)
     const int64_t n = tot->v < nq ? tot->v : nq;
-    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    // each XCD takes a contiguous eighth of the quads, so the vertex rows a quad shares with the
+    // quads of the previous plane are still in that XCD's L2
+    const int64_t q = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (q < n) edge_grad_one(quads, q, vinfo, adj, s_rcp, ge);
 }
 
