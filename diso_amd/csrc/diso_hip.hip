@@ -1885,7 +1885,7 @@ __global__ __launch_bounds__(kBlock) void dmc_edge_grad_batch_kernel(const Batch
     __shared__ S s_rcp[8];
     if (threadIdx.x < 8) s_rcp[threadIdx.x] = threadIdx.x ? S(1) / S(threadIdx.x) : S(0);
     __syncthreads();
-    const int64_t b = blockIdx.x;
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);  // as dmc_edge_grad_kernel
     const int j = find_group(rf, m, b);
     const BatchGrid<S> &G = grids[gl[j]];
     const int64_t q = (b - rf[j]) * kBlock + threadIdx.x;
