@@ -1954,18 +1954,33 @@ __device__ __forceinline__ S nan_max(S a, S b) {  // torch.max / np.maximum: NaN
     return (a != a || a > b) ? a : b;
 }
 
+// The unit vectors of a quad's six vertex pairs, e[i][j] = unit(p_j - p_i), the others by
+// negation: unit(-v) = -unit(v) exactly (same norm, sign-symmetric division), and a dot product
+// of negated vectors is the negated dot product exactly, so each corner cosine below has the bits
+// of normalising its two edge vectors itself (24 normalisations per quad become 6).
 template <typename S>
-__device__ __forceinline__ S corner_cos(V3<S> o, V3<S> a, V3<S> b) {
-    const V3<S> u = unit(V3<S>{a.x - o.x, a.y - o.y, a.z - o.z});
-    const V3<S> w = unit(V3<S>{b.x - o.x, b.y - o.y, b.z - o.z});
-    return u.x * w.x + u.y * w.y + u.z * w.z;
-}
-
-// Largest corner cosine of triangle (p0, p1, p2) (diso/__init__.py:127-136).
-template <typename S>
-__device__ __forceinline__ S tri_worst(V3<S> p0, V3<S> p1, V3<S> p2) {
-    return nan_max(nan_max(corner_cos(p0, p1, p2), corner_cos(p1, p2, p0)), corner_cos(p2, p0, p1));
-}
+struct QuadUnits {
+    V3<S> e[4][4];
+    __device__ __forceinline__ QuadUnits(const V3<S> *p) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = i + 1; j < 4; ++j) {
+                e[i][j] = unit(V3<S>{p[j].x - p[i].x, p[j].y - p[i].y, p[j].z - p[i].z});
+                e[j][i] = V3<S>{-e[i][j].x, -e[i][j].y, -e[i][j].z};
+            }
+    }
+    // cosine at corner o between the edges to a and to b (u first, then w, left-to-right dot)
+    __device__ __forceinline__ S cosine(int o, int a, int b) const {
+        const V3<S> &u = e[o][a], &w = e[o][b];
+        return u.x * w.x + u.y * w.y + u.z * w.z;
+    }
+    // largest corner cosine of triangle (a, b, c), corners in the order a, b, c
+    // (diso/__init__.py:127-136)
+    __device__ __forceinline__ S worst(int a, int b, int c) const {
+        return nan_max(nan_max(cosine(a, b, c), cosine(b, c, a)), cosine(c, a, b));
+    }
+};
 
 template <typename S>
 __device__ __forceinline__ V3<S> load_vert(const S *__restrict__ v, int64_t i) {
@@ -1991,10 +2006,11 @@ __global__ __launch_bounds__(kSplitBlock) void split_score_kernel(const S *__res
         }
         uint8_t f = 0;
         if (ok) {
-            const V3<S> p0 = load_vert(verts, id[0]), p1 = load_vert(verts, id[1]), p2 = load_vert(verts, id[2]),
-                        p3 = load_vert(verts, id[3]);
-            const S s1 = nan_max(tri_worst(p0, p1, p3), tri_worst(p1, p2, p3));
-            const S s2 = nan_max(tri_worst(p0, p1, p2), tri_worst(p0, p2, p3));
+            const V3<S> p[4] = {load_vert(verts, id[0]), load_vert(verts, id[1]), load_vert(verts, id[2]),
+                                load_vert(verts, id[3])};
+            const QuadUnits<S> qu(p);
+            const S s1 = nan_max(qu.worst(0, 1, 3), qu.worst(1, 2, 3));
+            const S s2 = nan_max(qu.worst(0, 1, 2), qu.worst(0, 2, 3));
             f = s1 < s2;
         }
         first[q] = f;
