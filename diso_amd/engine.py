@@ -53,8 +53,42 @@ def _ptr(t):
     return ctypes.c_void_p(0 if t is None else t.data_ptr())
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _stream(device):
+    """The current HIP stream of `device` as a raw handle (the direct query when torch has it:
+    the public one costs some ten microseconds per call)."""
+    if _raw_stream is not None:
+        return ctypes.c_void_p(_raw_stream(device.index if device.index is not None else torch.cuda.current_device()))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+_hiprt = None
+
+
+def _hip():
+    """The HIP runtime torch runs on (same soname, so the already loaded one), for the few
+    stream and event calls of the totals read-back."""
+    global _hiprt
+    if _hiprt is None:
+        h = ctypes.CDLL("libamdhip64.so.7")
+        vp, i32 = ctypes.c_void_p, ctypes.c_int
+        h.hipEventCreateWithFlags.argtypes = [ctypes.POINTER(vp), ctypes.c_uint]
+        h.hipEventRecord.argtypes = [vp, vp]
+        h.hipStreamWaitEvent.argtypes = [vp, vp, ctypes.c_uint]
+        h.hipMemcpyAsync.argtypes = [vp, vp, ctypes.c_size_t, i32, vp]
+        h.hipStreamSynchronize.argtypes = [vp]
+        for f in (h.hipEventCreateWithFlags, h.hipEventRecord, h.hipStreamWaitEvent, h.hipMemcpyAsync,
+                  h.hipStreamSynchronize):
+            f.restype = i32
+        _hiprt = h
+    return _hiprt
+
+
+def _hip_check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"diso_amd: {what} failed (hipError {rc})")
 
 
 class Extractor:
@@ -66,7 +100,7 @@ class Extractor:
         self.code = _DTYPES[dtype]
         self._ctl = {}
         self._guess = {}  # (device, dims, flags) -> (vertices, faces) of the last call
-        self._side = {}  # device -> (side stream, pinned totals)
+        self._side = {}  # device -> (side stream handle, pinned totals, event, side stream)
         _lib.lib()  # fail loudly now if the HIP library is not there
 
     def check_grid(self, grid, name="grid"):
@@ -135,12 +169,17 @@ class Extractor:
             _lib.check(min(nseg, off), "workspace sizing")
         seg = torch.empty(nseg, dtype=torch.uint8, device=dev)
         ctl = self.ctl_for(dev, dims, flags)
-        main = torch.cuda.current_stream(dev)
+        main = _stream(dev)
         rc = L.diso_count(self.algo, self.code, _ptr(grid), *dims, float(iso), flags, _ptr(seg), nseg, _ptr(ctl),
-                          ctl.numel(), None, ctypes.c_void_p(main.cuda_stream))
+                          ctl.numel(), None, main)
         _lib.check(rc, "count pass")
-        counted_ev = torch.cuda.Event()
-        counted_ev.record(main)
+        # the side stream copies the totals as soon as the count kernel is done
+        side, host, ev = self._side_for(dev)
+        hip = _hip()
+        _hip_check(hip.hipEventRecord(ev, main), "event record")
+        _hip_check(hip.hipStreamWaitEvent(side, ev, 0), "stream wait")
+        _hip_check(hip.hipMemcpyAsync(ctypes.c_void_p(host.data_ptr()), ctypes.c_void_p(seg.data_ptr() + off), 32,
+                                      2, side), "totals copy")  # 2: device to host
         dmc = self.algo == _lib.DMC
         vcap, fcap = guess[0] + guess[0] // 4 + 1024, guess[1] + guess[1] // 4 + 1024
         idx = torch.int64 if flags & _lib.FLAG_INDEX64 else torch.int32
@@ -148,18 +187,10 @@ class Extractor:
         faces = torch.empty((fcap, 4 if dmc else 3), dtype=idx, device=dev)
         vinfo = torch.empty(vcap if dmc else 0, dtype=torch.uint8, device=dev)
         rc = L.diso_emit(self.algo, self.code, _ptr(grid), _ptr(deform), *dims, float(iso), flags, _ptr(seg), nseg,
-                         _ptr(verts), vcap, _ptr(faces), fcap, _ptr(vinfo), ctypes.c_void_p(main.cuda_stream))
+                         _ptr(verts), vcap, _ptr(faces), fcap, _ptr(vinfo), main)
         _lib.check(rc, "emit pass")
-        side, host = self._side.get(dev.index, (None, None))
-        if side is None:
-            side = torch.cuda.Stream(device=dev)  # default priority
-            host = torch.empty(4, dtype=torch.int64, pin_memory=True)
-            self._side[dev.index] = (side, host)
-        side.wait_event(counted_ev)
-        with torch.cuda.stream(side):
-            host.copy_(seg[off:off + 32].view(torch.int64), non_blocking=True)
-        side.synchronize()
-        crossing, other, tflags, _ = (int(x) for x in host.tolist())
+        _hip_check(hip.hipStreamSynchronize(side), "totals wait")
+        crossing, other, tflags, _ = host.tolist()
         tot = _lib.Totals()
         tot.n_verts, tot.n_faces = (other, crossing) if dmc else (crossing, other)
         tot.flags, tot.n_crossing = tflags, crossing
@@ -171,6 +202,20 @@ class Extractor:
             counted.spec = (verts, faces, vinfo)
         self._remember(key, tot.n_verts, tot.n_faces)
         return counted
+
+    def _side_for(self, dev):
+        """Per device: a side stream (default priority), a pinned 32-byte landing buffer and an
+        event without timing, created once."""
+        got = self._side.get(dev.index)
+        if got is None:
+            side = torch.cuda.Stream(device=dev)
+            host = torch.empty(4, dtype=torch.int64, pin_memory=True)
+            ev = ctypes.c_void_p()
+            with torch.cuda.device(dev):
+                _hip_check(_hip().hipEventCreateWithFlags(ctypes.byref(ev), 2), "event create")  # 2: no timing
+            got = (ctypes.c_void_p(side.cuda_stream), host, ev, side)
+            self._side[dev.index] = got
+        return got[:3]
 
     def _remember(self, key, n_verts, n_faces):
         if len(self._guess) >= 64 and key not in self._guess:
