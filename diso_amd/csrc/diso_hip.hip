@@ -8,20 +8,28 @@
 //   (used cells in increasing linear cell id, src/cumc.cu:299-311,370-410,564-610). So the output
 //   position of any cell's first vertex / face is
 //       base(segment) + exclusive prefix, within the wave, of the per-cell counts,
-//   and only the per-segment bases are stored (16 bytes per 64 cells). Every kernel recomputes
-//   per-cell classifications from the sdf (an L1/L2-resident 3x3 row neighbourhood) and takes the
-//   in-wave prefix from bit-plane ballots + popcounts.
+//   and only the per-segment bases are stored (16 bytes per 64 cells). The count pass also stores
+//   one 16-bit classification word per position (code, owned crossings, inside bit, face count),
+//   which the later passes read instead of re-classifying (the recompute-classification backward
+//   re-derives the crossings from the samples instead). In-wave prefixes come from bit-plane
+//   ballots and v_mbcnt.
 //
 // Kernels
-//   count_kernel    classify + count + single-pass decoupled look-back scan of segment counts.
-//   emit_mc_kernel  MC vertices and triangles (+ shift / normalise / int64 epilogue).
-//   emit_dmc_kernel DMC vertices (patch means) and quads.
-//   bwd_mc_kernel   MC gradient: one thread per grid vertex gathers its <= 6 incident edges.
-//   bwd_dmc_kernel  DMC gradient: as above, each edge first sums its <= 4 incident patch adjoints.
+//   count_kernel          classify + count + single-pass decoupled look-back scan of segment
+//                         counts (also over the segments of a batch of grids).
+//   emit_mc_kernel        MC vertices and triangles (+ shift / normalise / int64 epilogue).
+//   emit_dmc_kernel       DMC vertices (patch means) and quads.
+//   dmc_edge_grad_kernel  DMC backward, first step: per crossing edge, the sum of its <= 4 quad
+//                         corners' vertex gradients / patch sizes.
+//   bwd_mc_kernel         MC gather backward (DMC: on those edge gradients): one lane per grid
+//                         vertex sums its <= 6 incident edges, deterministic, no atomics.
+//   split_*_kernel        DiffDMC's quad -> triangle split.
+//   *_batch_kernel        the same passes over many grids in one launch.
 //
-// Floating point: only IEEE +, -, *, / and compares, in the reference's operation order, built
-// with -ffp-contract=off and correctly rounded division, so fp32 results equal an fp32 CPU
-// evaluation of the same expressions bit for bit.
+// Floating point: forward results use only IEEE +, -, *, /, sqrt and compares, in the
+// reference's operation order, built with -ffp-contract=off and correctly rounded division, so
+// fp32 vertices equal an fp32 CPU evaluation of the same expressions bit for bit. The fp32
+// backward uses one hardware reciprocal per edge (gradients are compared within a tolerance).
 #include <hip/hip_runtime.h>
 
 #include <stdarg.h>
