@@ -846,6 +846,36 @@ __device__ __forceinline__ uint64_t run_active(const SegBase *__restrict__ seg, 
     return __ballot(s < hi && seg[s + 1].f != seg[s].f);
 }
 
+// The segments of a run with work (bits of `act`, segment lo + bit), taken by the block's waves
+// from a shared counter as each becomes free (their costs differ, so a fixed round-robin deal
+// leaves waves idle while the block, and its LDS, stays resident). A wave loads its next
+// segment (load(bit)) before it processes the current one (work(loaded)). *s_take must be zero
+// and visible to every wave of the block on entry.
+template <typename Load, typename Work>
+__device__ __forceinline__ void run_segments(uint64_t act, uint32_t *s_take, int lane, Load load, Work work) {
+    const int nact = __popcll(act);
+    auto grab = [&]() {
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(s_take, 1u);  // LDS
+        return __builtin_amdgcn_readfirstlane((int)k);
+    };
+    auto kth = [&](int k) {  // bit of the k-th active segment
+        const bool hit = ((act >> lane) & 1) && lanes_below(act) == k;
+        return (int)__builtin_ctzll(__ballot(hit));
+    };
+    int k = grab();
+    if (k >= nact) return;  // wave-uniform
+    auto nxt = load(kth(k));
+    for (;;) {
+        const auto cur = nxt;
+        k = grab();
+        const bool more = k < nact;  // wave-uniform
+        if (more) nxt = load(kth(k));
+        work(cur);
+        if (!more) break;
+    }
+}
+
 struct OutCfg {
     int flags;
     int64_t vcap, fcap;  // capacities in rows
@@ -1038,34 +1068,22 @@ __global__ __launch_bounds__(kBlock, 7) void emit_mc_kernel(In<S> in, const SegB
                                                          OutCfg out, Norm<S> nm, int run, int64_t nblocks) {
     __shared__ uint64_t s_case[256];
     __shared__ McEmitLds s_lds[kWavesPerBlock];
+    __shared__ uint32_t s_take;
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     for (int i = tid; i < 256; i += kBlock) s_case[i] = kMcCase[i];
+    if (tid == 0) s_take = 0;
     __syncthreads();
     const Geo &g = in.geo;
     const SegCtx sc{seg, info_of(seg, g), 0, 0};
     const SegBase *tot = seg + g.nseg;
     if (tot[0].v > out.vcap || tot[0].f > out.fcap) return;
     const int64_t lo = xcd_block(blockIdx.x, nblocks) * run, hi = lo + run < g.nseg ? lo + run : g.nseg;
-    // segments of the run with triangles (hence owned crossings), dealt round-robin to the waves
-    uint64_t act = run_active(seg, lo, hi, lane);
+    // segments of the run with triangles (hence owned crossings)
+    const uint64_t act = run_active(seg, lo, hi, lane);
     const SegPos lp = seg_pos(g, lo + lane);
-    // this wave's segments: every kWavesPerBlock-th active one; loads run one segment ahead
-    uint64_t mine = 0;
-    for (int k = 0; act; ++k, act &= act - 1)
-        if ((k & (kWavesPerBlock - 1)) == wave) mine |= act & (0ull - act);
-    if (mine) {
-        int i = __builtin_ctzll(mine);
-        McSegIn nxt = mc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
-        while (mine) {
-            const McSegIn cur = nxt;
-            mine &= mine - 1;
-            if (mine) {
-                i = __builtin_ctzll(mine);
-                nxt = mc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
-            }
-            emit_mc_segment<S, DEF>(in, sc, cur, verts, faces, out, nm, s_case, s_lds[wave], lane);
-        }
-    }
+    run_segments(
+        act, &s_take, lane, [&](int i) { return mc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane); },
+        [&](const McSegIn &d) { emit_mc_segment<S, DEF>(in, sc, d, verts, faces, out, nm, s_case, s_lds[wave], lane); });
 }
 
 // DMC emission. Patch-vertex item = (lane, patch): the mean of the MC vertices on the patch's
@@ -1369,32 +1387,21 @@ __global__ __launch_bounds__(kBlock, 6) void emit_dmc_kernel(In<S> in, const Seg
                                                           uint8_t *__restrict__ vinfo, OutCfg out, Norm<S> nm,
                                                           int run, int64_t nblocks) {
     __shared__ DmcEmitLds<S> s_lds[kWavesPerBlock];
+    __shared__ uint32_t s_take;
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
+    if (tid == 0) s_take = 0;
+    __syncthreads();
     const Geo &g = in.geo;
     const SegCtx sc{seg, info_of(seg, g), 0, 0};
     const SegBase *tot = seg + g.nseg;
     if (tot[0].v > out.fcap || tot[0].f > out.vcap) return;
     const int64_t lo = xcd_block(blockIdx.x, nblocks) * run, hi = lo + run < g.nseg ? lo + run : g.nseg;
-    // segments of the run with patches (hence owned crossings), dealt round-robin to the waves
-    uint64_t act = run_active(seg, lo, hi, lane);
+    // segments of the run with patches (hence owned crossings)
+    const uint64_t act = run_active(seg, lo, hi, lane);
     const SegPos lp = seg_pos(g, lo + lane);
-    // this wave's segments: every kWavesPerBlock-th active one; loads run one segment ahead
-    uint64_t mine = 0;
-    for (int k = 0; act; ++k, act &= act - 1)
-        if ((k & (kWavesPerBlock - 1)) == wave) mine |= act & (0ull - act);
-    if (mine) {
-        int i = __builtin_ctzll(mine);
-        DmcSegIn nxt = dmc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
-        while (mine) {
-            const DmcSegIn cur = nxt;
-            mine &= mine - 1;
-            if (mine) {
-                i = __builtin_ctzll(mine);
-                nxt = dmc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
-            }
-            emit_dmc_segment<S, DEF>(in, sc, cur, verts, faces, vinfo, out, nm, s_lds[wave], lane);
-        }
-    }
+    run_segments(
+        act, &s_take, lane, [&](int i) { return dmc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane); },
+        [&](const DmcSegIn &d) { emit_dmc_segment<S, DEF>(in, sc, d, verts, faces, vinfo, out, nm, s_lds[wave], lane); });
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1662,7 +1669,9 @@ __device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, c
 }
 
 // The touched segments of a block (bits of `act`, segment lo + bit) dealt round-robin over its
-// waves, each wave's next segment loaded while it gathers the current one.
+// waves, each wave's next segment loaded while it gathers the current one. (Taking them from a
+// shared counter, as the emission kernels do, measured slower here: the gathers' costs vary
+// less.)
 template <typename S, bool DEF, bool RECOMP>
 __device__ __forceinline__ void bwd_run(const In<S> &in, const SegCtx &sc, int64_t lo, const SegPos &lp, uint64_t act,
                                         const AdjIn<S> &adj, S *__restrict__ adj_sdf, S *__restrict__ adj_deform,
@@ -1816,8 +1825,10 @@ __global__ __launch_bounds__(kBlock, 7) void emit_mc_batch_kernel(const BatchGri
                                                                const uint16_t *__restrict__ info, int run, int64_t nblocks) {
     __shared__ uint64_t s_case[256];
     __shared__ McEmitLds s_lds[kWavesPerBlock];
+    __shared__ uint32_t s_take;
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     for (int i = tid; i < 256; i += kBlock) s_case[i] = kMcCase[i];
+    if (tid == 0) s_take = 0;
     __syncthreads();
     const int64_t b = xcd_block(blockIdx.x, nblocks);
     const int j = find_group(rf, m, b);
@@ -1827,25 +1838,13 @@ __global__ __launch_bounds__(kBlock, 7) void emit_mc_batch_kernel(const BatchGri
     const SegCtx sc = batch_ctx(G, seg, info);
     const OutCfg out{G.flags, G.vcap, G.fcap};
     const int64_t lo = (b - rf[j]) * run, hi = lo + run < in.geo.nseg ? lo + run : in.geo.nseg;
-    uint64_t act = run_active(sc.seg, lo, hi, lane);
+    const uint64_t act = run_active(sc.seg, lo, hi, lane);
     const SegPos lp = seg_pos(in.geo, lo + lane);
-    // this wave's segments: every kWavesPerBlock-th active one; loads run one segment ahead
-    uint64_t mine = 0;
-    for (int k = 0; act; ++k, act &= act - 1)
-        if ((k & (kWavesPerBlock - 1)) == wave) mine |= act & (0ull - act);
-    if (mine) {
-        int i = __builtin_ctzll(mine);
-        McSegIn nxt = mc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
-        while (mine) {
-            const McSegIn cur = nxt;
-            mine &= mine - 1;
-            if (mine) {
-                i = __builtin_ctzll(mine);
-                nxt = mc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
-            }
-            emit_mc_segment<S, DEF>(in, sc, cur, G.verts, G.faces, out, G.nm, s_case, s_lds[wave], lane);
-        }
-    }
+    run_segments(
+        act, &s_take, lane, [&](int i) { return mc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane); },
+        [&](const McSegIn &d) {
+            emit_mc_segment<S, DEF>(in, sc, d, G.verts, G.faces, out, G.nm, s_case, s_lds[wave], lane);
+        });
 }
 
 template <typename S, bool DEF>
@@ -1855,7 +1854,10 @@ __global__ __launch_bounds__(kBlock, 6) void emit_dmc_batch_kernel(const BatchGr
                                                                 const SegBase *__restrict__ seg,
                                                                 const uint16_t *__restrict__ info, int run, int64_t nblocks) {
     __shared__ DmcEmitLds<S> s_lds[kWavesPerBlock];
+    __shared__ uint32_t s_take;
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
+    if (tid == 0) s_take = 0;
+    __syncthreads();
     const int64_t b = xcd_block(blockIdx.x, nblocks);
     const int j = find_group(rf, m, b);
     const BatchGrid<S> &G = grids[gl[j]];
@@ -1864,24 +1866,13 @@ __global__ __launch_bounds__(kBlock, 6) void emit_dmc_batch_kernel(const BatchGr
     const SegCtx sc = batch_ctx(G, seg, info);
     const OutCfg out{G.flags, G.vcap, G.fcap};
     const int64_t lo = (b - rf[j]) * run, hi = lo + run < in.geo.nseg ? lo + run : in.geo.nseg;
-    uint64_t act = run_active(sc.seg, lo, hi, lane);
+    const uint64_t act = run_active(sc.seg, lo, hi, lane);
     const SegPos lp = seg_pos(in.geo, lo + lane);
-    uint64_t mine = 0;
-    for (int k = 0; act; ++k, act &= act - 1)
-        if ((k & (kWavesPerBlock - 1)) == wave) mine |= act & (0ull - act);
-    if (mine) {
-        int i = __builtin_ctzll(mine);
-        DmcSegIn nxt = dmc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
-        while (mine) {
-            const DmcSegIn cur = nxt;
-            mine &= mine - 1;
-            if (mine) {
-                i = __builtin_ctzll(mine);
-                nxt = dmc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane);
-            }
-            emit_dmc_segment<S, DEF>(in, sc, cur, (S *)G.verts, G.faces, G.vinfo, out, G.nm, s_lds[wave], lane);
-        }
-    }
+    run_segments(
+        act, &s_take, lane, [&](int i) { return dmc_load<S>(in, sc, lo + i, pos_of_lane(lp, i), lane); },
+        [&](const DmcSegIn &d) {
+            emit_dmc_segment<S, DEF>(in, sc, d, (S *)G.verts, G.faces, G.vinfo, out, G.nm, s_lds[wave], lane);
+        });
 }
 
 // Per-edge upstream gradients of every DMC grid of the batch: blocks of kBlock quads of one grid.
