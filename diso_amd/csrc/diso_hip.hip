@@ -834,9 +834,10 @@ __global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCou
 // the segment's k-th output row, so consecutive lanes write consecutive rows (coalesced stores).
 
 // Emission and backward blocks each take a run of `run` (<= 64) consecutive segments (so per-block
-// setup, such as the case tables in LDS, is paid once per run) and deal its segments with work
-// round-robin to their waves; the rest cost one lane-parallel load of the run's bases. The run
-// length is a launch parameter (diso_set_param).
+// setup, such as the case tables in LDS, is paid once per run) and share its segments with work
+// among their waves (emission: from a shared counter, run_segments; backward: round-robin); the
+// rest cost one lane-parallel load of the run's bases. The run length is a launch parameter
+// (diso_set_param).
 constexpr int kMaxRun = kWave;
 
 // Bit i: segment lo + i has faces (MC triangles / DMC patches), i.e. seg[s + 1].f != seg[s].f
