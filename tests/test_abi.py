@@ -181,3 +181,27 @@ def test_control_blocks_are_cleared_after_a_scan_timeout():
     ex.reset_ctl()
     assert all(int(c.sum()) == 0 for c in ex._ctl.values())
     assert _lib.E_TIMEOUT == -4
+
+
+def test_batch_descriptor_columns_alias_the_structs():
+    """The batch path reads and writes descriptor fields as numpy columns over the ctypes array:
+    the record layout must be GridT's, field for field (totals flattened)."""
+    from diso_amd.batch import _GRID_NP, _table, GridT
+
+    assert _GRID_NP.itemsize == ctypes.sizeof(GridT)
+    d = (GridT * 3)()
+    a = _table(d)
+    a["algo"] = [0, 1, 0]
+    a["flags"] = [1, 3, 7]
+    a["X"], a["Y"], a["Z"] = [5, 6, 7], [8, 9, 10], [11, 12, 13]
+    a["iso"] = 0.25
+    a["sdf"] = [0x1000, 0x2000, 1 << 47]
+    a["deform"] = [0, 0x3000, 0]
+    a["vert_capacity"] = [100, 200, 300]
+    a["adj_deform"] = [0, 0, 0x4000]
+    assert (d[1].algo, d[2].flags, d[0].X, d[1].Y, d[2].Z) == (1, 7, 5, 9, 13)
+    assert d[0].iso == 0.25 and d[2].sdf == 1 << 47 and d[0].deform is None and d[1].deform == 0x3000
+    assert d[2].vert_capacity == 300 and d[2].adj_deform == 0x4000
+    d[1].totals.n_verts, d[1].totals.n_faces, d[1].totals.flags, d[1].totals.n_crossing = 11, 22, 3, 44
+    assert (int(a["t_n_verts"][1]), int(a["t_n_faces"][1]), int(a["t_flags"][1]), int(a["t_n_crossing"][1])) == \
+        (11, 22, 3, 44)
