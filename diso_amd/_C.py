@@ -17,7 +17,7 @@ holds (a ring of samples >= iso around the grid, which the wrapper's padding gua
 import torch
 
 from . import _lib
-from .engine import Extractor
+from .engine import Extractor, check_backward_shapes
 
 
 class _Binding:
@@ -71,6 +71,7 @@ class _Binding:
             self._check_contiguous(adj_deform, "adj_deform")
         if self._last is None:
             raise RuntimeError("backward called before forward")
+        check_backward_shapes(self._last.dims, grid, deform, adj_grid, adj_deform)
         flags = _lib.FLAG_ACCUMULATE | (_lib.FLAG_REFERENCE_COMPAT if self.reference_compat else 0)
         with torch.cuda.device(grid.device):
             self._ex.backward(grid.contiguous(), None if deform is None else deform.contiguous(), self._last,

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libdiso_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "diso_hip.hip")]
-HEADERS = [os.path.join(ROOT, "include", "diso_hip.h"), os.path.join(HERE, "csrc", "diso_tables.h")]
+HEADERS = [os.path.join(ROOT, "include", "diso_hip.h"), os.path.join(HERE, "csrc", "diso_tables.h"),
+           os.path.join(HERE, "csrc", "diso_scan_word.h")]
 
 # Flags: gfx950 only; no FP contraction and correctly rounded division so results match an IEEE
 # evaluation of the reference's expressions; packed-fp32 instructions disabled.
@@ -34,10 +35,13 @@ TOTAL_HAS_BELOW = 1 << 0
 TOTAL_HAS_ABOVE = 1 << 1
 TOTAL_HAS_NAN = 1 << 2
 TOTAL_SCAN_TIMEOUT = 1 << 3
-E_TIMEOUT = -4  # DISO_E_TIMEOUT: the control block must be zero-filled before its next use
+E_TIMEOUT = -4  # DISO_E_TIMEOUT: the single-pass scan did not complete (never expected)
 PARAM_RUN = 0
 PARAM_COUNT_BLOCKS_PER_CU = 1
 PARAM_RECOUNT_SPINS = 2
+PARAM_BRICK_X = 3
+PARAM_BRICK_Y = 4
+PARAM_BRICK_Z = 5
 
 EXPORTS = [
     "diso_seg_bytes", "diso_ctl_bytes", "diso_scratch_bytes", "diso_count", "diso_emit", "diso_backward",

@@ -94,8 +94,6 @@ class _BatchExtract(torch.autograd.Function):
             ctl = st.ex.ctl_for(dev, n)
             rc = L.diso_batch_count(code, ctypes.addressof(d), n, _ptr(st.seg), nbytes, _ptr(ctl), ctl.numel(),
                                     stream)
-            if rc == _lib.E_TIMEOUT:
-                ctl.zero_()
             _lib.check(rc, "batch count pass")
         nbytes = st.seg.numel()
         both = _lib.TOTAL_HAS_BELOW | _lib.TOTAL_HAS_ABOVE
@@ -273,8 +271,6 @@ class BatchExtractor:
             ctl = self.ctl_for(dev, st.n)
             rc = L.diso_batch_count(self.code, ctypes.addressof(d), st.n, _ptr(st.seg), nbytes, _ptr(ctl),
                                     ctl.numel(), _stream(dev))
-            if rc == _lib.E_TIMEOUT:
-                ctl.zero_()
             _lib.check(rc, "batch count pass")
         a = st.table
         sizes = list(zip(a["t_n_verts"].tolist(), a["t_n_faces"].tolist(), a["t_flags"].tolist()))

@@ -53,6 +53,15 @@ def _ptr(t):
     return ctypes.c_void_p(0 if t is None else t.data_ptr())
 
 
+def _prefix(buf, n):
+    """The first n rows of `buf` as a tensor of their own over the same memory. Not a view (as
+    buf[:n] would be): autograd forbids in-place edits of views made inside a custom Function's
+    forward, and callers may edit the returned vertices in place."""
+    size = (n,) + tuple(buf.shape[1:])
+    return torch.empty(0, dtype=buf.dtype, device=buf.device).set_(buf.untyped_storage(), buf.storage_offset(), size,
+                                                                    buf.stride())
+
+
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
 
 
@@ -118,17 +127,11 @@ class Extractor:
         key = device.index
         ctl = self._ctl.get(key)
         if ctl is None or ctl.numel() < need:
-            # Grown buffers start zero-filled; the scan keeps them consistent from then on.
+            # Grown buffers start zero-filled; the scan keeps them consistent from then on (its
+            # words carry their call's epoch, so not even a timed-out call needs a reset).
             ctl = torch.zeros(max(need, 1 << 16) * 5 // 4, dtype=torch.uint8, device=device)
             self._ctl[key] = ctl
         return ctl
-
-    def reset_ctl(self):
-        """Zero-fills the control blocks (queued on the current streams). Needed only after a scan
-        timeout: a block that started after its call had given up on it may have left a word in
-        the buffer the next call expects to be clean."""
-        for ctl in self._ctl.values():
-            ctl.zero_()
 
     def count(self, grid, iso, flags):
         dims = tuple(int(d) for d in grid.shape)
@@ -141,8 +144,6 @@ class Extractor:
         tot = _lib.Totals()
         rc = L.diso_count(self.algo, self.code, _ptr(grid), *dims, float(iso), flags, _ptr(seg), nseg,
                           _ptr(ctl), ctl.numel(), ctypes.byref(tot), _stream(grid.device))
-        if rc == _lib.E_TIMEOUT:
-            self.reset_ctl()
         _lib.check(rc, "count pass")
         return Counted(seg, tot, dims, float(iso), flags)
 
@@ -195,7 +196,6 @@ class Extractor:
         tot.n_verts, tot.n_faces = (other, crossing) if dmc else (crossing, other)
         tot.flags, tot.n_crossing = tflags, crossing
         if tflags & _lib.TOTAL_SCAN_TIMEOUT:
-            self.reset_ctl()
             raise RuntimeError("diso_amd: count pass failed: single-pass scan did not complete")
         counted = Counted(seg, tot, dims, float(iso), flags)
         if tot.n_verts <= vcap and tot.n_faces <= fcap:
@@ -234,11 +234,11 @@ class Extractor:
         dmc = self.algo == _lib.DMC
         k = 4 if dmc else 3
         idx = torch.int64 if flags & _lib.FLAG_INDEX64 else torch.int32
-        if counted.spec is not None:  # emitted already (count_speculative): exact-size views
+        if counted.spec is not None:  # emitted already (count_speculative): exact-size prefixes
             v, f, vi = counted.spec
             counted.spec = None
-            verts, faces = v[:counted.n_verts], f[:counted.n_faces]
-            vinfo = vi[:counted.n_verts] if dmc else vi
+            verts, faces = _prefix(v, counted.n_verts), _prefix(f, counted.n_faces)
+            vinfo = _prefix(vi, counted.n_verts) if dmc else vi
         else:
             verts = torch.empty((counted.n_verts, 3), dtype=self.dtype, device=grid.device)
             faces = torch.empty((counted.n_faces, k), dtype=idx, device=grid.device)
@@ -258,6 +258,7 @@ class Extractor:
 
     def backward(self, grid, deform, counted, flags, adj_verts, adj_grid, adj_deform):
         """Writes (or, with FLAG_ACCUMULATE, adds) d(sum(adj_verts*verts))/d(grid, deform)."""
+        check_backward_shapes(counted.dims, grid, deform, adj_grid, adj_deform)
         L = _lib.lib()
         adj_verts = adj_verts.contiguous()
         if adj_verts.dtype != self.dtype:
@@ -277,6 +278,24 @@ class Extractor:
                              _ptr(counted.vinfo if dmc else None), _ptr(adj_verts), counted.n_verts,
                              _ptr(adj_grid), _ptr(adj_deform), _ptr(scratch), nscratch, _stream(grid.device))
         _lib.check(rc, "backward pass")
+
+
+def check_backward_shapes(dims, grid, deform, adj_grid, adj_deform):
+    """The backward writes dims worth of gradients: the grid, the gradient buffers and (with
+    deform) the deform must have the forward's dims, or the kernels would address memory outside
+    them. Raises RuntimeError (the reference binding's error type) otherwise."""
+    dims = tuple(int(d) for d in dims)
+    if tuple(grid.shape) != dims:
+        raise RuntimeError(f"grid has shape {tuple(grid.shape)}, the forward's was {dims}")
+    if adj_grid is None or tuple(adj_grid.shape) != dims:
+        raise RuntimeError(f"adj_grid must have shape {dims}, got {None if adj_grid is None else tuple(adj_grid.shape)}")
+    if deform is not None:
+        want = dims + (3,)
+        if tuple(deform.shape) != want:
+            raise RuntimeError(f"deform must have shape {want}, got {tuple(deform.shape)}")
+        if adj_deform is None or tuple(adj_deform.shape) != want:
+            raise RuntimeError(f"adj_deform must have shape {want}, got "
+                               f"{None if adj_deform is None else tuple(adj_deform.shape)}")
 
 
 def split_quads(verts, quads):

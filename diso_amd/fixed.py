@@ -143,7 +143,6 @@ class FixedExtractor:
         if fl & _lib.TOTAL_HAS_NAN:
             raise RuntimeError("diso_amd: sdf grid contains NaN values")
         if fl & _lib.TOTAL_SCAN_TIMEOUT:
-            self.ex.reset_ctl()
             raise RuntimeError("diso_amd: single-pass scan did not complete")
         if nv > self.vert_capacity or nf > self.face_capacity:
             raise RuntimeError(f"diso_amd: {nv} vertices / {nf} faces exceed the capacities "

@@ -53,8 +53,8 @@ enum {
     DISO_E_INVALID = -1,  /* bad argument                                                   */
     DISO_E_HIP = -2,      /* HIP runtime error                                              */
     DISO_E_CAPACITY = -3, /* a workspace or output buffer is too small                      */
-    DISO_E_TIMEOUT = -4   /* the single-pass scan did not complete (never expected); zero-fill
-                             the control block before its next use                          */
+    DISO_E_TIMEOUT = -4   /* the single-pass scan did not complete (never expected); the
+                             control block stays usable: no reset is needed                */
 };
 
 /* Status bits of diso_totals_t.flags. */
@@ -197,11 +197,22 @@ int diso_batch_backward(int dtype, const diso_grid_t *grids, int n, void *seg, i
 /* Launch parameters, process-wide (set them before launching work; the autotuner in
  * diso_amd/autotune.py picks them per machine). Results do not depend on them.
  *  DISO_PARAM_RUN: segments per emission / backward block, 1..64 (default 32).
- *  DISO_PARAM_COUNT_BLOCKS_PER_CU: resident blocks per CU of the count pass, 1..4 (default 4).
- *  DISO_PARAM_RECOUNT_SPINS: reads of a predecessor's unpublished aggregate before a count block
- *    counts that range itself, 0..2^20 (default 8192; 0 recounts every predecessor not already
- *    published, which tests use to exercise that path). */
-enum { DISO_PARAM_RUN = 0, DISO_PARAM_COUNT_BLOCKS_PER_CU = 1, DISO_PARAM_RECOUNT_SPINS = 2 };
+ *  DISO_PARAM_COUNT_BLOCKS_PER_CU: resident blocks per CU of the count pass's scan, 1..4
+ *    (default 4).
+ *  DISO_PARAM_RECOUNT_SPINS: reads of a predecessor's unpublished aggregate before a scan block
+ *    sums that range itself, 0..2^20 (default 8192; 0 sums every predecessor not already
+ *    published, which tests use to exercise that path).
+ *  DISO_PARAM_BRICK_X / _Y / _Z: the classification brick, rows along x and y (1..32) and
+ *    64-position chunks along z (1..32), default 8 x 8 x 8; refused when its staged words would
+ *    exceed 64 KB of LDS. Results do not depend on any parameter. */
+enum {
+    DISO_PARAM_RUN = 0,
+    DISO_PARAM_COUNT_BLOCKS_PER_CU = 1,
+    DISO_PARAM_RECOUNT_SPINS = 2,
+    DISO_PARAM_BRICK_X = 3,
+    DISO_PARAM_BRICK_Y = 4,
+    DISO_PARAM_BRICK_Z = 5
+};
 int diso_set_param(int which, int value);
 int diso_get_param(int which);
 

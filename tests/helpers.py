@@ -42,3 +42,25 @@ def grad_close(got, want, rel):
     err = float(np.abs(got.astype(np.float64) - want.astype(np.float64)).max())
     assert err <= rel * scale, f"max abs err {err:.3e} > {rel:.1e} * scale {scale:.3e}"
     return err / scale
+
+
+def check_against_oracle(algo, sdf_np, def_np, verts, faces, up=None, grad_sdf=None, grad_deform=None, iso=0.0,
+                         normalize=True, return_quads=True, compat=False):
+    """An extraction's outputs (torch tensors) against the oracle on the same inputs: faces and
+    vertices bit-exact, gradients (for upstream `up`) within 2e-5 (fp32) / 1e-12 (fp64) of the
+    gradient's scale."""
+    from oracle import oracle as O
+
+    dtype = sdf_np.dtype.type
+    ref = O.Extractor(O.MC if algo in ("mc", O.MC) else O.DMC, dtype)
+    rv, rf = ref.forward(sdf_np, def_np, iso, normalize, return_quads=return_quads)
+    assert tuple(faces.shape) == rf.shape, (tuple(faces.shape), rf.shape)
+    assert np.array_equal(faces.cpu().numpy(), rf), "faces differ from the oracle"
+    assert np.array_equal(verts.detach().cpu().numpy(), rv), "vertices differ from the oracle"
+    if up is None or ref.empty:
+        return
+    tol = 2e-5 if dtype == np.float32 else 1e-12
+    gs, gd = ref.backward(up.detach().cpu().numpy().astype(dtype), reference_compat=compat)
+    grad_close(grad_sdf.cpu().numpy(), gs, tol)
+    if def_np is not None:
+        grad_close(grad_deform.cpu().numpy(), gd, tol)

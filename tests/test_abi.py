@@ -157,7 +157,17 @@ def test_launch_parameters():
         assert L.diso_set_param(_lib.PARAM_RUN, 0) == -1 and L.diso_set_param(_lib.PARAM_RUN, 65) == -1
         assert L.diso_set_param(_lib.PARAM_COUNT_BLOCKS_PER_CU, 5) == -1 and L.diso_set_param(9, 1) == -1
         autotune.apply({"run": 16, "count_blocks_per_cu": 2})
-        assert autotune.current() == {"run": 16, "count_blocks_per_cu": 2}
+        assert autotune.current() == {"run": 16, "count_blocks_per_cu": 2, "brick": before["brick"]}
+        # brick shapes: extents 1..32, staged LDS at most 64 KB; any valid shape is reachable
+        assert before["brick"] == [8, 8, 8]
+        for bad in (0, 33):
+            assert L.diso_set_param(_lib.PARAM_BRICK_X, bad) == -1 and L.diso_set_param(_lib.PARAM_BRICK_Z, bad) == -1
+        for shape in ([32, 32, 1], [1, 1, 32], [16, 4, 3], [8, 8, 8]):
+            autotune.apply({"brick": shape})
+            assert autotune.current()["brick"] == shape
+        autotune.apply({"brick": [32, 8, 8]})
+        assert L.diso_set_param(_lib.PARAM_BRICK_Y, 32) == -1  # 32 x 32 x 8 would stage 35 x 35 x 10 x 16 B
+        assert b"stages more than" in L.diso_last_error() and autotune.current()["brick"] == [32, 8, 8]
         with pytest.raises(RuntimeError):
             autotune.apply({"run": 100, "count_blocks_per_cu": 2})
         spins = L.diso_get_param(_lib.PARAM_RECOUNT_SPINS)
@@ -168,19 +178,6 @@ def test_launch_parameters():
         assert L.diso_set_param(_lib.PARAM_RECOUNT_SPINS, spins) == 0
     finally:
         autotune.apply(before)
-
-
-def test_control_blocks_are_cleared_after_a_scan_timeout():
-    """After a scan timeout the extractor zero-fills its control blocks (a late block may have left
-    a word in the buffer the next call expects clean); the plain count path does it when the
-    library reports DISO_E_TIMEOUT."""
-    from diso_amd.engine import Extractor
-
-    ex = Extractor(_lib.MC, torch.float32)
-    ex._ctl = {("cpu", 1): torch.full((64,), 7, dtype=torch.uint8), ("cpu", 2): torch.ones(16, dtype=torch.uint8)}
-    ex.reset_ctl()
-    assert all(int(c.sum()) == 0 for c in ex._ctl.values())
-    assert _lib.E_TIMEOUT == -4
 
 
 def test_batch_descriptor_columns_alias_the_structs():
@@ -205,3 +202,38 @@ def test_batch_descriptor_columns_alias_the_structs():
     d[1].totals.n_verts, d[1].totals.n_faces, d[1].totals.flags, d[1].totals.n_crossing = 11, 22, 3, 44
     assert (int(a["t_n_verts"][1]), int(a["t_n_faces"][1]), int(a["t_flags"][1]), int(a["t_n_crossing"][1])) == \
         (11, 22, 3, 44)
+
+
+def test_backward_shape_checks():
+    """A backward whose grid, deform or gradient buffers do not have the forward's dims is refused
+    before any kernel runs (the kernels write the forward's dims worth of gradients)."""
+    from diso_amd.engine import check_backward_shapes
+
+    dims = (4, 5, 6)
+    g, d = torch.zeros(dims), torch.zeros(dims + (3,))
+    check_backward_shapes(dims, g, d, torch.zeros(dims), torch.zeros(dims + (3,)))
+    check_backward_shapes(dims, g, None, torch.zeros(dims), None)
+    bad = [
+        (torch.zeros(4, 5, 7), d, torch.zeros(dims), torch.zeros(dims + (3,))),  # grid of other dims
+        (g, d, torch.zeros(4, 5, 5), torch.zeros(dims + (3,))),  # smaller adj_grid
+        (g, d, torch.zeros(dims), torch.zeros(4, 5, 6, 2)),  # adj_deform of the wrong shape
+        (g, d, torch.zeros(dims), None),  # no adj_deform with a deform
+        (g, torch.zeros(4, 5, 6, 2), torch.zeros(dims), torch.zeros(dims + (3,))),  # deform shape
+        (g, None, None, None),  # no adj_grid
+    ]
+    for args in bad:
+        with pytest.raises(RuntimeError):
+            check_backward_shapes(dims, *args)
+
+
+def test_reference_import_names_resolve():
+    """`from diso import DiffMC, DiffDMC` and `diso._C.CUMCFloat` (the reference's import names)
+    resolve to this package's classes."""
+    import diso
+    import diso._C
+    import diso_amd
+    import diso_amd._C
+
+    assert diso.DiffMC is diso_amd.DiffMC and diso.DiffDMC is diso_amd.DiffDMC
+    for name in ("CUMCFloat", "CUMCDouble", "CUDMCFloat", "CUDMCDouble"):
+        assert getattr(diso._C, name) is getattr(diso_amd._C, name)

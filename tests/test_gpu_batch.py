@@ -1,13 +1,13 @@
-"""Batched extraction against per-grid extraction (needs a GPU): every grid of a mixed batch
-(MC and DMC, uneven and odd dims, with and without deform, an empty grid) gives bit-identical
-vertices, faces and gradients to DiffMC / DiffDMC on that grid alone, which the per-grid parity
-tests tie to the oracle."""
+"""Batched extraction against per-grid extraction and the oracle (needs a GPU): every grid of a
+mixed batch (MC and DMC, uneven and odd dims, with and without deform, an empty grid) gives
+bit-identical vertices, faces and gradients to DiffMC / DiffDMC on that grid alone, and equals the
+oracle (faces and vertices bit-exact, gradients within the parity tolerance)."""
 import numpy as np
 import pytest
 import torch
 
 import diso_amd
-from tests.helpers import inputs
+from tests.helpers import check_against_oracle, inputs
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -71,6 +71,9 @@ def test_batch_matches_per_grid(dtype, compat, recompute, return_quads):
                 assert torch.equal(defs[i].grad, pgd), f"grid {i} deform gradient"
         else:
             assert sdfs[i].grad is None or not sdfs[i].grad.any()
+        if algos[i] == "mc" or return_quads:  # (the split's cut is tie-tolerant, tested elsewhere)
+            check_against_oracle(algos[i], s_np, d_np, v, f, ups[i], sdfs[i].grad,
+                                 None if defs[i] is None else defs[i].grad, compat=compat)
 
 
 def test_batch_twice_and_nan():

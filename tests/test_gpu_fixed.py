@@ -12,7 +12,7 @@ import torch
 import diso_amd
 from diso_amd import FixedExtractor
 from diso_amd.fit import SdfFit
-from tests.helpers import inputs
+from tests.helpers import check_against_oracle, inputs
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -78,8 +78,23 @@ def test_fixed_overflow_writes_nothing():
 
 
 @pytest.mark.parametrize("algo", ["mc", "dmc"])
-def test_captured_step_replays_match_eager(algo):
-    print(f"case graph replay {algo}", flush=True)
+@pytest.mark.parametrize("recount_spins", [None, 0])
+def test_captured_step_replays_match_eager(algo, recount_spins):
+    """recount_spins 0: every count block counts the ranges of predecessors that have not yet
+    published instead of waiting; replays must stay exact (no host step between them resets
+    anything)."""
+    print(f"case graph replay {algo} recount_spins={recount_spins}", flush=True)
+    L = diso_amd._lib.lib()
+    spins = L.diso_get_param(diso_amd._lib.PARAM_RECOUNT_SPINS)
+    if recount_spins is not None:
+        assert L.diso_set_param(diso_amd._lib.PARAM_RECOUNT_SPINS, recount_spins) == 0
+    try:
+        _replays(algo)
+    finally:
+        L.diso_set_param(diso_amd._lib.PARAM_RECOUNT_SPINS, spins)
+
+
+def _replays(algo):
     dims = (40, 36, 44)
     fields = [inputs(dims, kind, seed, np.float32, True) for seed, kind in enumerate(["noise", "sphere", "blobs"])]
     cap_v = 4 * 42 * 38 * 46  # more than any grid of these dims can produce (padded positions x 4)
@@ -120,6 +135,8 @@ def test_captured_step_replays_match_eager(algo):
         assert torch.equal(f_s[:nf], f_e[:nf]), k
         assert torch.equal(sdf.grad, gs_e), k
         assert torch.equal(deform.grad, gd_e), k
+        # and the replay against the oracle
+        check_against_oracle(algo, sdf_np, def_np, v_s[:nv], f_s[:nf], w[:nv], sdf.grad, deform.grad)
 
 
 def test_captured_fit_matches_eager_fit():

@@ -12,53 +12,14 @@ path is tied to size-independent properties instead.
 import numpy as np
 import pytest
 import torch
-import torch.nn.functional as F
 
 import diso_amd
 from diso_amd import FixedExtractor, mesh, workloads
 from oracle import oracle as O
+from tests.torch_counts import expected_counts
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
-
-MC_CORNERS = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
-
-
-def _padded_inside(sdf, iso=0.0):
-    return F.pad(sdf, (1, 1, 1, 1, 1, 1), value=iso + 1.0) >= iso
-
-
-def _crossings(ins):
-    return sum(int((ins.narrow(a, 0, ins.shape[a] - 1) != ins.narrow(a, 1, ins.shape[a] - 1)).sum()) for a in range(3))
-
-
-def _codes(ins, corners):
-    X, Y, Z = (d - 1 for d in ins.shape)
-    code = torch.zeros((X, Y, Z), dtype=torch.int64, device=ins.device)
-    for i, (dx, dy, dz) in enumerate(corners):
-        code |= ins[dx:dx + X, dy:dy + Y, dz:dz + Z].long() << i
-    return code
-
-
-def _good_codes(code, pc):
-    """The face-neighbour rule (src/cudualmc.cu:815-839) on a whole code grid."""
-    d = pc[code]
-    out = code.clone()
-    dims = code.shape
-    for comp in range(3):
-        for step in (1, -1):
-            sel = d == (2 * comp + (1 if step > 0 else 0))
-            if not bool(sel.any()):
-                continue
-            idx = sel.nonzero(as_tuple=True)
-            n = [t.clone() for t in idx]
-            n[comp] = n[comp] + step
-            ok = (n[comp] >= 0) & (n[comp] < dims[comp])
-            nb = torch.full_like(idx[0], 255)
-            nb[ok] = pc[code[n[0][ok], n[1][ok], n[2][ok]]]
-            flip = ok & (nb != 255)
-            out[idx[0][flip], idx[1][flip], idx[2][flip]] ^= 255
-    return out
 
 
 def test_counts_at_256_match_independent_counts():
@@ -66,12 +27,7 @@ def test_counts_at_256_match_independent_counts():
     tables = {k: torch.tensor(v, device=DEV, dtype=torch.int64) for k, v in O.expand_tables().items()}
     sdf_np, def_np, _ = workloads.noisy_round_box(256, seed=0)
     sdf, deform = torch.tensor(sdf_np, device=DEV), torch.tensor(def_np, device=DEV)
-    ins = _padded_inside(sdf)
-    nx = _crossings(ins)
-    mc_code = _codes(ins, MC_CORNERS)
-    ntri = int(((tables["fmi"][1:] - tables["fmi"][:-1]) // 3)[mc_code].sum())
-    dmc_code = _good_codes(_codes(ins, [(i & 1, (i >> 1) & 1, (i >> 2) & 1) for i in range(8)]), tables["pc"])
-    npatch = int((tables["fpi"][1:] - tables["fpi"][:-1])[dmc_code].sum())
+    nx, ntri, npatch = expected_counts(sdf, tables)
     v, f = diso_amd.DiffMC()(sdf, deform)
     assert v.shape[0] == nx and f.shape[0] == ntri
     assert int(f.min()) >= 0 and int(f.max()) < nx

@@ -248,9 +248,10 @@ def test_results_do_not_depend_on_launch_parameters():
         # recount spins 0: every count block counts the ranges of the predecessors that have not
         # published when it first looks (the path that keeps the scan live when launches share
         # the device)
-        for run, cpc, rs in ((1, 1, spins), (7, 3, 0), (32, 4, spins), (64, 2, 0)):
-            print(f"case launch run={run} count_blocks_per_cu={cpc} recount_spins={rs}", flush=True)
-            autotune.apply({"run": run, "count_blocks_per_cu": cpc})
+        for run, cpc, rs, brick in ((1, 1, spins, (1, 1, 1)), (7, 3, 0, (3, 5, 2)), (32, 4, spins, (8, 8, 8)),
+                                    (64, 2, 0, (32, 32, 1)), (16, 4, spins, (16, 4, 3))):
+            print(f"case launch run={run} count_blocks_per_cu={cpc} recount_spins={rs} brick={brick}", flush=True)
+            autotune.apply({"run": run, "count_blocks_per_cu": cpc, "brick": list(brick)})
             assert L.diso_set_param(diso_amd._lib.PARAM_RECOUNT_SPINS, rs) == 0
             res = []
             for mod in (diso_amd.DiffMC(), diso_amd.DiffDMC()):
@@ -346,3 +347,26 @@ def test_backward_makes_no_face_gradient(algo):
     peak = torch.cuda.max_memory_allocated() - base
     grads = (sdf.numel() + deform.numel()) * 4
     assert peak < grads + scratch + face_bytes // 2, (peak, grads, scratch, face_bytes)
+
+
+@pytest.mark.parametrize("algo", [O.MC, O.DMC])
+def test_reused_module_outputs_allow_inplace_edits(algo):
+    """A module's second call on grids of the same dims returns the speculatively emitted rows;
+    those are tensors of their own, so in-place edits of the vertices under autograd work as on
+    the first call, and the gradient flows through the edited vertices."""
+    print(f"case inplace edit algo={algo}", flush=True)
+    sdf_np, def_np = inputs((24, 26, 30), "noisy_box", 2, np.float32, True)
+    mod = _module(algo, np.float32)
+    kw = {"return_quads": True} if algo == O.DMC else {}
+    grads = []
+    for call in range(3):
+        sdf = torch.tensor(sdf_np, device=DEV, requires_grad=True)
+        deform = torch.tensor(def_np, device=DEV, requires_grad=True)
+        v, f = mod(sdf, deform, **kw)
+        v[:, 0] += 1.0
+        v.mul_(2.0)
+        f += 0
+        (v * v).sum().backward()
+        grads.append((v.detach().clone(), sdf.grad, deform.grad))
+    for v, gs, gd in grads[1:]:
+        assert torch.equal(v, grads[0][0]) and torch.equal(gs, grads[0][1]) and torch.equal(gd, grads[0][2])

@@ -1,6 +1,6 @@
 """Serving loop (needs a GPU): requests submitted from several threads and batched by two
-workers give the same results, bit for bit, as each request extracted alone; a tight memory
-budget serialises the work without changing results."""
+workers give the same results, bit for bit, as each request extracted alone, and equal the
+oracle; a tight memory budget serialises the work without changing results."""
 import threading
 
 import numpy as np
@@ -9,7 +9,7 @@ import torch
 
 import diso_amd
 from diso_amd.serve import Request, Server
-from tests.helpers import inputs
+from tests.helpers import check_against_oracle, inputs
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -71,6 +71,12 @@ def test_served_results_equal_alone(budget, workers):
             assert torch.equal(res[2], gs)
             if gd is not None:
                 assert torch.equal(res[3], gd)
+        sdf, deform, algo, backward, seed = a
+        up = None
+        if gs is not None:
+            up = torch.randn((v.shape[0], 3), generator=torch.Generator(device=DEV).manual_seed(seed), device=DEV)
+        check_against_oracle(algo, sdf.cpu().numpy(), None if deform is None else deform.cpu().numpy(), res[0], res[1],
+                             up, None if up is None else res[2], None if up is None else res[3])
 
 
 def test_concurrent_streams_match_sequential():
