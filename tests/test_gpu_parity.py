@@ -349,15 +349,18 @@ def test_backward_makes_no_face_gradient(algo):
     assert peak < grads + scratch + face_bytes // 2, (peak, grads, scratch, face_bytes)
 
 
-@pytest.mark.parametrize("algo", [O.MC, O.DMC])
-def test_reused_module_outputs_allow_inplace_edits(algo):
+@pytest.mark.parametrize("algo,quads", [(O.MC, False), (O.DMC, False), (O.DMC, True)])
+def test_reused_module_outputs_allow_inplace_edits(algo, quads):
     """A module's second call on grids of the same dims returns the speculatively emitted rows;
-    those are tensors of their own, so in-place edits of the vertices under autograd work as on
-    the first call, and the gradient flows through the edited vertices."""
-    print(f"case inplace edit algo={algo}", flush=True)
+    those are tensors of their own, so in-place edits of the vertices (and of MC / split DMC
+    triangles) under autograd work as on the first call, and the gradient flows through the
+    edited vertices. DiffDMC's quads with return_quads=True are what its backward reads: an
+    in-place edit of them makes the backward raise autograd's saved-tensor error (never a
+    silently wrong gradient)."""
+    print(f"case inplace edit algo={algo} quads={quads}", flush=True)
     sdf_np, def_np = inputs((24, 26, 30), "noisy_box", 2, np.float32, True)
     mod = _module(algo, np.float32)
-    kw = {"return_quads": True} if algo == O.DMC else {}
+    kw = {"return_quads": quads} if algo == O.DMC else {}
     grads = []
     for call in range(3):
         sdf = torch.tensor(sdf_np, device=DEV, requires_grad=True)
@@ -365,8 +368,14 @@ def test_reused_module_outputs_allow_inplace_edits(algo):
         v, f = mod(sdf, deform, **kw)
         v[:, 0] += 1.0
         v.mul_(2.0)
-        f += 0
-        (v * v).sum().backward()
+        if quads:
+            (v * v).sum().backward(retain_graph=True)
+            f += 0
+            with pytest.raises(RuntimeError, match="inplace"):
+                (v * v).sum().backward()
+        else:
+            f += 0
+            (v * v).sum().backward()
         grads.append((v.detach().clone(), sdf.grad, deform.grad))
     for v, gs, gd in grads[1:]:
         assert torch.equal(v, grads[0][0]) and torch.equal(gs, grads[0][1]) and torch.equal(gd, grads[0][2])

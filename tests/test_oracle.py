@@ -235,6 +235,34 @@ def test_dmc_reference_compat_differs_only_in_multi_patch_cells():
     assert not np.allclose(a, b)
 
 
+@pytest.mark.parametrize("with_deform", [False, True])
+def test_dmc_reference_compat_is_true_backward_of_first_patch_gradients(with_deform):
+    """The reference's DMC backward reads each cell's first patch-vertex gradient for every patch
+    of the cell (src/cudualmc.cu:990). So compat(g) equals the true backward of g', where g' copies
+    the gradient of each cell's first patch vertex to all of that cell's patch vertices. Pins
+    which patch compat reads (the first, not the last or its own) independently of the
+    compat code path; the vertex ranges come from the cells' codes and the patch-count table."""
+    sdf, deform = inputs((11, 10, 12), "noise", 5, np.float64, with_deform)
+    ex = O.Extractor(O.DMC, np.float64)
+    v, _ = ex.forward(sdf, deform, 0.0, True, return_quads=True)
+    codes, _ = ex.engine.codes()
+    npatch = np.diff(O.expand_tables()["fpi"])[codes]
+    assert npatch.min() >= 1 and npatch.sum() == v.shape[0]
+    assert npatch.max() >= 2  # multi-patch cells present
+    first = np.concatenate([[0], np.cumsum(npatch)[:-1]])
+    cell_of_vertex = np.repeat(np.arange(len(codes)), npatch)
+    g = np.random.default_rng(3).standard_normal(v.shape)
+    g1 = g[first[cell_of_vertex]]
+    a_s, a_d = ex.backward(g, reference_compat=True)
+    b_s, b_d = ex.backward(g1)
+    np.testing.assert_allclose(a_s, b_s, rtol=1e-12, atol=1e-12 * np.abs(b_s).max())
+    if with_deform:
+        np.testing.assert_allclose(a_d, b_d, rtol=1e-12, atol=1e-12 * np.abs(b_d).max())
+    # and g' differs from g, so the identity is not the trivial one
+    c_s, _ = ex.backward(g)
+    assert not np.allclose(a_s, c_s)
+
+
 def test_dmc_correct_mode_matches_fd_on_multi_patch_cells():
     """Finite differences on a field with multi-patch cells agree with the default backward."""
     rng = np.random.default_rng(11)
