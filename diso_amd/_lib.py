@@ -10,7 +10,9 @@ import torch  # noqa: F401  (loads torch's HIP runtime first, so the library bin
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-LIB_PATH = os.path.join(HERE, "libdiso_hip.so")
+# DISO_AMD_LIB selects another build of the same library (e.g. the host-sanitizer build the
+# tests make); by default the in-tree one.
+LIB_PATH = os.environ.get("DISO_AMD_LIB") or os.path.join(HERE, "libdiso_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "diso_hip.hip")]
 HEADERS = [os.path.join(ROOT, "include", "diso_hip.h"), os.path.join(HERE, "csrc", "diso_tables.h"),
            os.path.join(HERE, "csrc", "diso_scan_word.h")]
@@ -39,9 +41,6 @@ E_TIMEOUT = -4  # DISO_E_TIMEOUT: the single-pass scan did not complete (never e
 PARAM_RUN = 0
 PARAM_COUNT_BLOCKS_PER_CU = 1
 PARAM_RECOUNT_SPINS = 2
-PARAM_BRICK_X = 3
-PARAM_BRICK_Y = 4
-PARAM_BRICK_Z = 5
 
 EXPORTS = [
     "diso_seg_bytes", "diso_ctl_bytes", "diso_scratch_bytes", "diso_count", "diso_emit", "diso_backward",

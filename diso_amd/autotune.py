@@ -28,20 +28,12 @@ def apply(params):
     if "count_blocks_per_cu" in params:
         _lib.check(L.diso_set_param(_lib.PARAM_COUNT_BLOCKS_PER_CU, int(params["count_blocks_per_cu"])),
                    "set count blocks per CU")
-    if "brick" in params:
-        # the library refuses bricks that stage too much LDS, and its size grows with each extent:
-        # shrink first, then grow, so that every intermediate shape is valid when the final one is
-        keys = (_lib.PARAM_BRICK_X, _lib.PARAM_BRICK_Y, _lib.PARAM_BRICK_Z)
-        steps = sorted(((int(v) - L.diso_get_param(k), k, int(v)) for k, v in zip(keys, params["brick"])))
-        for _, which, v in steps:
-            _lib.check(L.diso_set_param(which, v), "set brick shape")
 
 
 def current():
     L = _lib.lib()
     return {"run": L.diso_get_param(_lib.PARAM_RUN),
-            "count_blocks_per_cu": L.diso_get_param(_lib.PARAM_COUNT_BLOCKS_PER_CU),
-            "brick": [L.diso_get_param(w) for w in (_lib.PARAM_BRICK_X, _lib.PARAM_BRICK_Y, _lib.PARAM_BRICK_Z)]}
+            "count_blocks_per_cu": L.diso_get_param(_lib.PARAM_COUNT_BLOCKS_PER_CU)}
 
 
 def load(path=TUNING_FILE):

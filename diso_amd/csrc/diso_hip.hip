@@ -485,27 +485,80 @@ __device__ __forceinline__ int find_group(const int64_t *__restrict__ rf, int m,
 }
 
 // ------------------------------------------------------------------------------------------
-// Pass 1, part 1: classification of LDS-staged bricks (the reference's per-cell classifiers,
-// count_used_cells_kernel and the per-used-cell count kernels, src/cumc.cu:265-341,540-562,
-// src/cudualmc.cu:605-681,815-863)
+// Pass 1: classify, count, scan (single pass)
 //
-// A block classifies one brick: a tile of TX x TY rows (px, py) of the grid and KZ chunks of 64 z
-// positions (launch parameters). It first stages in LDS, for the brick's rows plus a halo (MC:
-// +1 row in x and y, the cells' far corners; DMC: -1 and +2, the face neighbours of ambiguous
-// cells) and for its chunks plus the next one (DMC: and the previous one), two bit-words per 64
-// samples of a row: inside (sample >= iso) and below (sample < iso). Every staged sample is read
-// once, by a coalesced row load, compared once and packed by a ballot. Every segment of the
-// brick (one row's chunk) is then classified from those words alone:
-//   - the crossing masks of the x / y / z edges the positions own are word operations over rows
-//     (px, py), (px+1, py), (px, py+1) and the row shifted by one z, so a segment's vertex (MC)
-//     or quad (DMC) count is three popcounts;
-//   - a lane's cube code collects its bit of the eight corner words, and its face count (MC
-//     triangles, DMC patches) comes from a case table in LDS;
-//   - a DMC cell whose case is ambiguous reads its face neighbour's corners from the same words.
-// Outputs: the classification word of every position, each segment's counts (the scan turns
-// them into bases) and the grid's sample flags. Crossings follow the reference's test exactly,
-// (d0 < iso and d1 >= iso) or (d1 < iso and d0 >= iso), which is why "below" is kept besides
-// "inside" (a NaN sample is neither).
+// The grid of segments is split into G contiguous ranges, one per block (G ~ 4 blocks per CU).
+// A block counts its segments (writing each segment's counts into
+// `seg`), publishes its aggregate as one self-contained 64-bit word, reads the aggregates of all
+// its predecessors in parallel (a look-back over the whole window at once: no chain of inclusive
+// prefixes, so the wait is one memory latency rather than one per predecessor), and finally scans
+// its own segment counts into segment bases. Aggregate words are accessed with agent-scope
+// atomics only, so they are coherent across the XCDs' L2s without fences. Two word buffers
+// alternate between calls; each call clears the one the next call uses (up to the extent that
+// was written), and the last block advances the epoch once every block has published (and so has
+// read the epoch), so no initialisation launch is needed. Words are tagged with their call's
+// epoch (diso_scan_word.h: tag, sample flags, face / patch count, vertex / quad count), so a word
+// that a block of an earlier call wrote late never counts for a later call, and no reset is
+// needed after a timeout. A predecessor that does not publish in time (it may not be running:
+// other launches can fill the slots of the XCD it was dispatched to) is counted by the waiting
+// block itself; only the last block waits for all, and every block it waits for ends in bounded
+// time once dispatched.
+
+struct CtlHeader {
+    uint32_t epoch;     // calls completed on this block (diso_scan_word.h)
+    uint32_t dirty[2];  // words written in each buffer since it was last cleared
+    uint32_t pad[13];
+};
+
+constexpr int kCountWaves = 8;
+constexpr int kCountBlock = kCountWaves * kWave;
+constexpr int64_t kMaxCountBlocks = 8192;
+constexpr int kCountBlocksPerCU = 4;
+
+__device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Per-position counts for pass 1: nv = crossing edges owned (MC vertices / DMC quads);
+// nf = MC triangles / DMC patches; fl = sample flags of the position's own sample; the position's
+// classification word.
+template <typename S>
+using CornerWin = Win<S, 0, 2, 0, 2, 0, 2>;
+
+template <typename S>
+__device__ __forceinline__ void count_position(const In<S> &in, int ALGO, const uint64_t *cases,
+                                               const CornerWin<S> &w, int *nv, int *nf, uint32_t *fl,
+                                               uint32_t *info) {
+    const int px = w.px, py = w.py, pz = w.pz;
+    const S v = w(0, 0, 0);
+    *fl = in_data(in, px, py, pz) ? ((v < in.iso ? DISO_TOTAL_HAS_BELOW : 0u) |
+                                     (v > in.iso ? DISO_TOTAL_HAS_ABOVE : 0u) | (v != v ? DISO_TOTAL_HAS_NAN : 0u))
+                                  : 0u;
+    *nv = 0;
+    *nf = 0;
+    *info = (uint32_t)(v >= in.iso) << 11;
+    if (!is_anchor(in, px, py, pz)) return;
+    const int m = own_mask(v, w(1, 0, 0), w(0, 1, 0), w(0, 0, 1), in.iso);
+    *nv = popc3(m);
+    int code;
+    if (ALGO == DISO_MC) {
+        Corners<S> k;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) k.c[i & 1][(i >> 1) & 1][(i >> 2) & 1] = w(i & 1, (i >> 1) & 1, (i >> 2) & 1);
+        code = mc_code(k, in.iso);
+        *nf = DISO_MC_NTRI(cases[code]);
+    } else {
+        code = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) code |= (w(i & 1, (i >> 1) & 1, (i >> 2) & 1) >= in.iso) << i;
+        if (code != 0 && code != 255) code = dmc_good_code(in, cases, code, px, py, pz);
+        *nf = DISO_DMC_NPATCH(cases[code]);
+    }
+    *info |= (uint32_t)code | ((uint32_t)m << 8) | ((uint32_t)*nf << 12);
+}
 
 // Wave sum of a per-lane value in [0, 7] (three ballots).
 __device__ __forceinline__ int wave_total_small(int v) {
@@ -522,321 +575,89 @@ __device__ __forceinline__ int64_t wave_incl64(int64_t v) {
     return v;
 }
 
-struct Brick {
-    int tx, ty, kz;  // rows along x, rows along y, chunks along z
-};
-
-constexpr int kClassWaves = 4;
-constexpr int kClassBlock = kClassWaves * kWave;
-constexpr int kStageUnroll = 8;  // row loads in flight per wave while staging
+// What a count launch classifies: one grid (`in`, algorithm ALGO), or (BATCH) the concatenated
+// segments of several grids, each with its own descriptor; then per-grid sample flags are
+// or-ed into gflags[grid] (zero on entry) and the scan runs over all segments in order.
 
 template <typename S>
-struct ClassSrc {
-    In<S> in;            // the grid (single-grid launch)
-    int algo;            // single-grid launch
-    uint32_t *flags;     // single grid: its sample-flag word; batch: one word per grid
-    Brick br;
-    const int32_t *gl;   // batch: grid of each group
-    const int64_t *rf;   // batch: prefix of the groups' brick counts
-    int m;               // batch: number of groups
-};
-
-// The brick of a block and what it stages, for a grid of geometry g.
-struct BrickGeo {
-    int x0, y0, k0, nx, ny, nk;       // brick origin (row, row, chunk) and extent, clipped to the grid
-    int sx0, sy0, sk0, nrx, nry, nw;  // staged rows [sx0, sx0 + nrx) x [sy0, sy0 + nry), words [sk0, sk0 + nw)
-};
-
-__device__ __forceinline__ int64_t bricks_of(const Geo &g, const Brick &br) {
-    return (int64_t)((g.g0 + br.tx - 1) / br.tx) * ((g.g1 + br.ty - 1) / br.ty) * ((g.nchunk + br.kz - 1) / br.kz);
-}
-
-__device__ __forceinline__ BrickGeo brick_geo(const Geo &g, const Brick &br, int64_t t, bool dmc) {
-    const int ntz = (g.nchunk + br.kz - 1) / br.kz, nty = (g.g1 + br.ty - 1) / br.ty;
-    const int tz = (int)(t % ntz), ty = (int)((t / ntz) % nty), tx = (int)(t / ((int64_t)ntz * nty));
-    BrickGeo T;
-    T.x0 = tx * br.tx;
-    T.y0 = ty * br.ty;
-    T.k0 = tz * br.kz;
-    T.nx = min(br.tx, g.g0 - T.x0);
-    T.ny = min(br.ty, g.g1 - T.y0);
-    T.nk = min(br.kz, g.nchunk - T.k0);
-    const int lo = dmc ? 1 : 0, hi = dmc ? 2 : 1;
-    T.sx0 = max(T.x0 - lo, 0);
-    T.sy0 = max(T.y0 - lo, 0);
-    T.nrx = min(T.x0 + T.nx + hi, g.g0) - T.sx0;
-    T.nry = min(T.y0 + T.ny + hi, g.g1) - T.sy0;
-    T.sk0 = max(T.k0 - lo, 0);
-    T.nw = min(T.k0 + T.nk + 1, g.nchunk) - T.sk0;
-    return T;
-}
-
-// A 64-bit word that all lanes read from the same LDS address, moved to scalar registers.
-__device__ __forceinline__ uint64_t lds_uniform64(const uint64_t *p) {
-    const uint64_t v = *p;
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ uint32_t bit_at(uint64_t w, int i) { return (uint32_t)(w >> i) & 1u; }
-
-// The word of positions z + 1 (bits shifted down by one, the next word's bit 0 on top).
-__device__ __forceinline__ uint64_t next_z(uint64_t w, uint64_t wn) { return (w >> 1) | (wn << 63); }
-
-// Iterates (a, b, c) over [0, na) x [0, nb) x [0, nc), c fastest, starting at linear index
-// `first` and advancing by `step` (< nc * nb * na) without divisions per step.
-struct Cursor3 {
-    int a, b, c;
-    __device__ __forceinline__ Cursor3(int first, int nb, int nc) {
-        c = first % nc;
-        const int r = first / nc;
-        b = r % nb;
-        a = r / nb;
-    }
-    __device__ __forceinline__ void advance(int step, int nb, int nc) {
-        c += step;
-        while (c >= nc) {  // uniform
-            c -= nc;
-            if (++b == nb) {
-                b = 0;
-                ++a;
-            }
-        }
-    }
-};
-
-// Per-case entries the classification needs: face count (MC triangles / DMC patches) in bits
-// 0-2, the DMC ambiguity direction in bits 3-5 (7: not ambiguous).
-__device__ __forceinline__ uint8_t class_entry(uint64_t w, bool dmc) {
-    return dmc ? (uint8_t)(DISO_DMC_NPATCH(w) | (DISO_DMC_DIR(w) << 3)) : (uint8_t)(DISO_MC_NTRI(w) | (7 << 3));
-}
-
-template <typename S, bool BATCH>
-__global__ __launch_bounds__(kClassBlock) void classify_kernel(ClassSrc<S> src, const BatchGrid<S> *__restrict__ grids,
-                                                               SegBase *seg_all, uint16_t *info_all, int64_t nblocks) {
-    extern __shared__ uint64_t s_bits[];  // staged words: inside, then below
-    __shared__ uint8_t s_tab[2][256];
-    __shared__ uint32_t s_fl;
-    const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
-    for (int i = tid; i < 256; i += kClassBlock) {
-        s_tab[0][i] = class_entry(kMcCase[i], false);
-        s_tab[1][i] = class_entry(kDmcCase[i], true);
-    }
-    if (tid == 0) s_fl = 0;
-
-    const int64_t b = xcd_block(blockIdx.x, nblocks);
+struct CountSrc {
     In<S> in;
-    int algo;
-    int64_t t, first = 0;
-    uint32_t *flags_out;
-    if constexpr (BATCH) {
-        const int j = find_group(src.rf, src.m, b);
-        const int gi = src.gl[j];
-        in = load_uniform(&grids[gi].in);
-        algo = grids[gi].algo;
-        first = grids[gi].seg_first;
-        t = b - src.rf[j];
-        flags_out = src.flags + gi;
-    } else {
-        in = src.in;
-        algo = src.algo;
-        t = b;
-        flags_out = src.flags;
-    }
-    const bool dmc = algo == DISO_DMC;
-    const Geo &g = in.geo;
-    const BrickGeo T = brick_geo(g, src.br, t, dmc);
-    const int nstage = T.nrx * T.nry * T.nw;
-    uint64_t *s_in = s_bits, *s_bl = s_bits + nstage;
-
-    // 1. Stage the inside / below words: job = (row, word), word fastest; a wave takes every
-    // kClassWaves-th job and keeps kStageUnroll row loads in flight.
-    uint32_t fl = 0;
-    {
-        Cursor3 cur(wave, T.nry, T.nw);
-        for (int j0 = wave; j0 < nstage; j0 += kClassWaves * kStageUnroll) {
-            S v[kStageUnroll];
-            int jx[kStageUnroll], jy[kStageUnroll], jz[kStageUnroll];
-#pragma unroll
-            for (int u = 0; u < kStageUnroll; ++u) {
-                jx[u] = T.sx0 + cur.a;
-                jy[u] = T.sy0 + cur.b;
-                jz[u] = kWave * (T.sk0 + cur.c) + lane;
-                if (j0 + u * kClassWaves < nstage) v[u] = sample(in, jx[u], jy[u], jz[u]);
-                cur.advance(kClassWaves, T.nry, T.nw);
-            }
-#pragma unroll
-            for (int u = 0; u < kStageUnroll; ++u) {
-                const int j = j0 + u * kClassWaves;
-                if (j < nstage) {  // uniform
-                    const uint64_t bi = __ballot(v[u] >= in.iso), bb = __ballot(v[u] < in.iso);
-                    if (lane == 0) {
-                        s_in[j] = bi;
-                        s_bl[j] = bb;
-                    }
-                    if (in_data(in, jx[u], jy[u], jz[u]))
-                        fl |= (v[u] < in.iso ? DISO_TOTAL_HAS_BELOW : 0u) | (v[u] > in.iso ? DISO_TOTAL_HAS_ABOVE : 0u) |
-                              (v[u] != v[u] ? DISO_TOTAL_HAS_NAN : 0u);
-                }
-            }
-        }
-    }
-    fl = wave_or(fl);
-    __syncthreads();  // staged words and tables complete
-    if (lane == 0 && fl) atomicOr(&s_fl, fl);  // LDS
-
-    // 2. Classify the brick's segments: wave takes every kClassWaves-th, chunk fastest.
-    const uint8_t *tab = s_tab[dmc ? 1 : 0];
-    SegBase *seg = seg_all + first;
-    uint16_t *info = info_all + first * kWave;
-    const int nseg_brick = T.nx * T.ny * T.nk;
-    Cursor3 cur(wave, T.ny, T.nk);
-    for (int q = wave; q < nseg_brick; q += kClassWaves, cur.advance(kClassWaves, T.ny, T.nk)) {
-        const int px = T.x0 + cur.a, py = T.y0 + cur.b, k = T.k0 + cur.c;
-        const int z = kWave * k + lane;
-        // word index of (row px + dx, py + dy, chunk k + dk) in the staged arrays
-        const int r00 = ((px - T.sx0) * T.nry + (py - T.sy0)) * T.nw + (k - T.sk0);
-        const int rdx = T.nry * T.nw, rdy = T.nw;
-        const uint64_t I00 = lds_uniform64(s_in + r00);
-        uint32_t iw = bit_at(I00, lane) << 11;
-        int tv = 0, nf = 0;
-        if (px < g.g0 - 1 && py < g.g1 - 1) {  // the row anchors cells (uniform)
-            // the next chunk's words matter only when z + 1 of lane 63 is in the grid; then staged
-            const bool nxt = k + 1 < T.sk0 + T.nw;
-            const uint64_t I10 = lds_uniform64(s_in + r00 + rdx), I01 = lds_uniform64(s_in + r00 + rdy),
-                           I11 = lds_uniform64(s_in + r00 + rdx + rdy);
-            const uint64_t B00 = lds_uniform64(s_bl + r00), B10 = lds_uniform64(s_bl + r00 + rdx),
-                           B01 = lds_uniform64(s_bl + r00 + rdy);
-            uint64_t I00n = 0, I10n = 0, I01n = 0, I11n = 0, B00n = 0;
-            if (nxt) {
-                I00n = lds_uniform64(s_in + r00 + 1);
-                I10n = lds_uniform64(s_in + r00 + rdx + 1);
-                I01n = lds_uniform64(s_in + r00 + rdy + 1);
-                I11n = lds_uniform64(s_in + r00 + rdx + rdy + 1);
-                B00n = lds_uniform64(s_bl + r00 + 1);
-            }
-            // lanes anchoring a cell: z < g2 - 1
-            const int na = g.g2 - 1 - kWave * k;
-            const uint64_t A = na >= kWave ? ~0ull : (na <= 0 ? 0ull : (1ull << na) - 1);
-            const uint64_t I00z = next_z(I00, I00n), I10z = next_z(I10, I10n), I01z = next_z(I01, I01n),
-                           I11z = next_z(I11, I11n), B00z = next_z(B00, B00n);
-            const uint64_t cx = ((B00 & I10) | (B10 & I00)) & A;
-            const uint64_t cy = ((B00 & I01) | (B01 & I00)) & A;
-            const uint64_t cz = ((B00 & I00z) | (B00z & I00)) & A;
-            tv = __popcll(cx) + __popcll(cy) + __popcll(cz);
-            const uint32_t m = bit_at(cx, lane) | (bit_at(cy, lane) << 1) | (bit_at(cz, lane) << 2);
-            int code;
-            if (!dmc)  // Bourke corner order (src/cumc.cu:107)
-                code = (int)(bit_at(I00, lane) | (bit_at(I10, lane) << 1) | (bit_at(I11, lane) << 2) |
-                             (bit_at(I01, lane) << 3) | (bit_at(I00z, lane) << 4) | (bit_at(I10z, lane) << 5) |
-                             (bit_at(I11z, lane) << 6) | (bit_at(I01z, lane) << 7));
-            else  // Morton corner order (src/cudualmc.cu:95-104)
-                code = (int)(bit_at(I00, lane) | (bit_at(I10, lane) << 1) | (bit_at(I01, lane) << 2) |
-                             (bit_at(I11, lane) << 3) | (bit_at(I00z, lane) << 4) | (bit_at(I10z, lane) << 5) |
-                             (bit_at(I01z, lane) << 6) | (bit_at(I11z, lane) << 7));
-            if (!bit_at(A, lane)) code = 0;
-            int e = tab[code];
-            if (dmc && (e >> 3) != 7 && code != 0 && code != 255) {
-                // getGoodCellCode (src/cudualmc.cu:815-839): invert when the face neighbour in the
-                // tabulated direction exists and is ambiguous too; its corners are staged words
-                const int dir = e >> 3, comp = dir >> 1, step = (dir & 1) ? 1 : -1;
-                const int nx = px + (comp == 0 ? step : 0), ny = py + (comp == 1 ? step : 0),
-                          nz = z + (comp == 2 ? step : 0);
-                const int c = comp == 0 ? nx : (comp == 1 ? ny : nz);
-                const int lim = comp == 0 ? g.g0 : (comp == 1 ? g.g1 : g.g2);
-                if (c >= 0 && c < lim - 1) {
-                    int ncode = 0;
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const int cxr = nx + (i & 1) - T.sx0, cyr = ny + ((i >> 1) & 1) - T.sy0,
-                                  czz = nz + ((i >> 2) & 1);
-                        const uint64_t w = s_in[(cxr * T.nry + cyr) * T.nw + (czz >> 6) - T.sk0];
-                        ncode |= (int)bit_at(w, czz & 63) << i;
-                    }
-                    if ((tab[ncode] >> 3) != 7) {
-                        code ^= 0xff;
-                        e = tab[code];
-                    }
-                }
-            }
-            nf = e & 7;
-            iw |= (uint32_t)code | (m << 8) | ((uint32_t)nf << 12);
-        }
-        const int tf = wave_total_small(nf);
-        const int64_t s = seg_index(g, px, py, k);
-        info[s * kWave + lane] = (uint16_t)iw;
-        if (lane == 0) {
-            seg[s].v = tv;
-            seg[s].f = tf;
-        }
-    }
-    __syncthreads();
-    if (tid == 0 && s_fl) atomicOr(flags_out, s_fl);
-}
-
-// ------------------------------------------------------------------------------------------
-// Pass 1, part 2: scan of the segment counts into segment bases (single pass), replacing the
-// reference's three CUB exclusive scans and their host synchronisations (src/cumc.cu:668-723,
-// src/cudualmc.cu:1075-1121)
-//
-// The segments are split into G contiguous ranges, one per block (G ~ 4 blocks per CU). A block
-// sums its segments' counts, publishes the sum as one self-contained 64-bit word, reads the
-// words of all its predecessors in parallel (a look-back over the whole window at once: no
-// chain of inclusive prefixes, so the wait is one memory latency rather than one per
-// predecessor), and finally scans its own segment counts into segment bases. Words are accessed
-// with agent-scope atomics only, so they are coherent across the XCDs' L2s without fences. Two
-// word buffers alternate between calls; each call clears the one the next call uses (up to the
-// extent that was written), and the last block advances the epoch once every block has published
-// (and so has read the epoch), so no initialisation launch is needed. Words are tagged with their
-// call's epoch (diso_scan_word.h), so a word that a block of an earlier call wrote late never
-// counts for a later call, and no reset is needed after a timeout. A predecessor that does not
-// publish in time (it may not be running: other launches can fill the slots of the XCD it was
-// dispatched to) is summed by the waiting block itself from the counts; only the last block waits
-// for all, and every block it waits for ends in bounded time once dispatched. The last block
-// also writes the totals, with the grid's sample flags (single grid; a batch's per-grid flags
-// are read by batch_totals_kernel).
-
-struct CtlHeader {
-    uint32_t epoch;         // calls completed on this block (diso_scan_word.h)
-    uint32_t dirty[2];      // words written in each buffer since it was last cleared
-    uint32_t sample_flags;  // single grid: or of the classification blocks' sample flags
-    uint32_t pad[12];
-};
-
-constexpr int kCountWaves = 8;
-constexpr int kCountBlock = kCountWaves * kWave;
-constexpr int64_t kMaxCountBlocks = 8192;
-constexpr int kCountBlocksPerCU = 4;
-
-__device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_relaxed(uint64_t *p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-struct ScanSrc {
+    uint32_t *gflags;
+    int ngrids;
+    uint32_t recount_spins;  // reads of an unpublished predecessor word before counting its range
     int64_t nseg;            // segments of the launch
-    uint32_t recount_spins;  // reads of an unpublished predecessor word before summing its range
-    int algo;                // recorded in the totals (single grid)
 };
 
-// Sum of the counts of segments [lo, hi), wave-wide (every lane gets it).
-__device__ __forceinline__ void sum_counts(const SegBase *seg, int64_t lo, int64_t hi, int lane, int64_t &v,
-                                           int64_t &f) {
-    int64_t a = 0, c = 0;
-    for (int64_t s = lo + lane; s < hi; s += kWave) {
-        a += seg[s].v;
-        c += seg[s].f;
+// Counts segments [wlo, whi) (one wave, in order; its position advances without divisions) and
+// adds the wave's totals to (wv, wf, wfl). WRITE: also record each segment's counts (lane 0),
+// every lane's classification word and (BATCH) the grids' sample flags; without WRITE the range
+// is only summed (a block recounting a predecessor's range, see the look-back).
+template <typename S, int ALGO, bool BATCH, bool WRITE>
+__device__ __forceinline__ void count_range(const CountSrc<S> &src, const BatchGrid<S> *__restrict__ grids,
+                                            SegBase *seg, uint16_t *info, const uint64_t (*s_case)[256],
+                                            int64_t wlo, int64_t whi, int lane, int64_t &wv, int64_t &wf,
+                                            uint32_t &wfl) {
+    // BATCH: the range may span several grids; one pass of the outer loop per grid, with that
+    // grid's parameters held in scalar registers for the inner loop.
+    int gi = 0;
+    if constexpr (BATCH)
+        while (gi + 1 < src.ngrids && wlo >= grids[gi + 1].seg_first) ++gi;
+    for (int64_t s = wlo; s < whi;) {  // wave-uniform
+        In<S> in;
+        if constexpr (BATCH)
+            in = load_uniform(&grids[gi].in);
+        else
+            in = src.in;
+        const int algo = BATCH ? grids[gi].algo : ALGO;
+        const int64_t first = BATCH ? grids[gi].seg_first : 0;
+        const int64_t end = (BATCH && gi + 1 < src.ngrids && grids[gi + 1].seg_first < whi) ? grids[gi + 1].seg_first
+                                                                                          : whi;
+        const uint64_t *cases = s_case[(BATCH && algo != DISO_MC) ? 1 : 0];
+        uint32_t gfl = 0;  // the grid's sample flags
+        SegPos sp = seg_pos(in.geo, s - first);
+        for (; s < end; ++s) {
+            int nv, nf;
+            uint32_t fl, iw;
+            CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
+            w.template load_box<0, 2, 0, 2, 0, 2>(in);
+            count_position<S>(in, algo, cases, w, &nv, &nf, &fl, &iw);
+            if (++sp.chunk == in.geo.nchunk) {  // next segment of the grid
+                sp.chunk = 0;
+                if (++sp.py == in.geo.g1) {
+                    sp.py = 0;
+                    ++sp.px;
+                }
+            }
+            const int tv = wave_total_small(nv), tf = wave_total_small(nf);
+            if constexpr (WRITE) {
+                info[s * kWave + lane] = (uint16_t)iw;
+                if (lane == 0) {
+                    seg[s].v = tv;
+                    seg[s].f = tf;
+                }
+            }
+            wv += tv;
+            wf += tf;
+            gfl |= fl;
+        }
+        wfl |= gfl;
+        if constexpr (BATCH) {
+            if constexpr (WRITE) {
+                gfl = wave_or(gfl);
+                if (lane == 0 && gfl) atomicOr(src.gflags + gi, gfl);
+            }
+            ++gi;
+        }
     }
-    v = wave_sum64(a);
-    f = wave_sum64(c);
 }
 
-__global__ __launch_bounds__(kCountBlock, kCountBlocksPerCU * kCountWaves / 4) void scan_kernel(
-    ScanSrc src, SegBase *seg, uint8_t *ctl, int64_t nblocks) {
+template <typename S, int ALGO, bool BATCH>
+__global__ __launch_bounds__(kCountBlock, (BATCH ? 3 : kCountBlocksPerCU) * kCountWaves / 4) void count_kernel(
+    CountSrc<S> src, const BatchGrid<S> *__restrict__ grids, SegBase *seg, uint16_t *info, uint8_t *ctl,
+    int64_t nblocks) {
+    __shared__ uint64_t s_case[BATCH ? 2 : 1][256];
     __shared__ int64_t s_v[kCountWaves], s_f[kCountWaves];
+    __shared__ uint32_t s_fl[kCountWaves];
     __shared__ uint32_t s_epoch, s_dirty_other, s_next;
     __shared__ uint64_t s_word;
 
@@ -847,6 +668,14 @@ __global__ __launch_bounds__(kCountBlock, kCountBlocksPerCU * kCountWaves / 4) v
     const int64_t b = blockIdx.x;
     const int64_t lo = b * nseg / nblocks, hi = (b + 1) * nseg / nblocks;
 
+    for (int i = tid; i < 256; i += kCountBlock) {
+        if (BATCH) {
+            s_case[0][i] = kMcCase[i];
+            s_case[BATCH ? 1 : 0][i] = kDmcCase[i];
+        } else {
+            s_case[0][i] = ALGO == DISO_MC ? kMcCase[i] : kDmcCase[i];
+        }
+    }
     if (tid == 0) {
         const uint32_t ep = __hip_atomic_load(&hdr->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_epoch = ep;
@@ -858,32 +687,40 @@ __global__ __launch_bounds__(kCountBlock, kCountBlocksPerCU * kCountWaves / 4) v
     const uint32_t parity = diso_scan::buffer_of(epoch), tag = diso_scan::tag_of(epoch);
     uint64_t *cur = words + (int64_t)parity * kMaxCountBlocks;
 
-    // 1. Sum: wave w takes the w-th eighth of the block's segments.
-    int64_t wv, wf;
-    sum_counts(seg, lo + (hi - lo) * wave / kCountWaves, lo + (hi - lo) * (wave + 1) / kCountWaves, lane, wv, wf);
+    // 1. Count: wave w takes the w-th eighth of the block's segments, in order; lane 0 records
+    // each segment's counts, every lane its classification word.
+    int64_t wv = 0, wf = 0;
+    uint32_t wfl = 0;
+    count_range<S, ALGO, BATCH, true>(src, grids, seg, info, s_case, lo + (hi - lo) * wave / kCountWaves,
+                                      lo + (hi - lo) * (wave + 1) / kCountWaves, lane, wv, wf, wfl);
+    wfl = wave_or(wfl);
     if (lane == 0) {
         s_v[wave] = wv;
         s_f[wave] = wf;
+        s_fl[wave] = wfl;
     }
     __syncthreads();
     int64_t av = 0, af = 0;
+    uint32_t afl = 0;
 #pragma unroll
     for (int w = 0; w < kCountWaves; ++w) {
         av += s_v[w];
         af += s_f[w];
+        afl |= s_fl[w];
     }
-    if (tid == 0) st_relaxed(cur + b, diso_scan::pack(av, af, 0, tag));
+    if (tid == 0) st_relaxed(cur + b, diso_scan::pack(av, af, afl, tag));
 
-    // 2. Exclusive prefix of this block = sum of all predecessors' words. A predecessor that has
-    // not published within `recount_spins` reads is not waited for any longer: this block sums
-    // that range itself (read-only; the same sums), so no block waits on one that may not be
-    // running (several launches sharing the device fill the XCDs' slots). The last block alone
+    // 2. Exclusive prefix of this block = sum of all predecessors' aggregates. A predecessor that
+    // has not published within `recount_spins` reads is not waited for any longer: this block
+    // counts that range itself (read-only; the same sums), so no block waits on one that may not
+    // be running (several launches sharing the device fill the XCDs' slots). The last block alone
     // waits for every predecessor's word: a block publishes after reading the epoch, so once all
-    // have published the last block may advance it; every other block ends in bounded time, so
-    // the blocks it waits for are all dispatched.
+    // have published the last block may advance it; every other block ends in bounded time, so the
+    // blocks it waits for are all dispatched.
     const bool last = b == nblocks - 1;
     const uint32_t spin_limit = last ? kSpinLimit / (uint32_t)(kMaxCountBlocks / kCountBlock) : src.recount_spins;
     int64_t pv = 0, pf = 0;
+    uint32_t pfl = 0;
     uint32_t miss = 0;  // bit k: predecessor tid + k * kCountBlock did not publish in time
     for (int64_t j = tid, k = 0; j < b; j += kCountBlock, ++k) {
         uint64_t w;
@@ -896,14 +733,17 @@ __global__ __launch_bounds__(kCountBlock, kCountBlocksPerCU * kCountWaves / 4) v
         } else {
             pv += diso_scan::count_v(w);
             pf += diso_scan::count_f(w);
+            pfl |= diso_scan::flags_of(w);
         }
     }
     pv = wave_sum64(pv);
     pf = wave_sum64(pf);
+    pfl = wave_or(pfl);
     __syncthreads();  // s_v / s_f reuse
     if (lane == 0) {
         s_v[wave] = pv;
         s_f[wave] = pf;
+        s_fl[wave] = pfl;
     }
     __syncthreads();
     int64_t ev = 0, ef = 0;
@@ -912,6 +752,7 @@ __global__ __launch_bounds__(kCountBlock, kCountBlocksPerCU * kCountWaves / 4) v
     for (int w = 0; w < kCountWaves; ++w) {
         ev += s_v[w];
         ef += s_f[w];
+        efl |= s_fl[w];
     }
     // the missing predecessors, lowest first, one at a time by the whole block
     for (;;) {
@@ -929,34 +770,38 @@ __global__ __launch_bounds__(kCountBlock, kCountBlocksPerCU * kCountWaves / 4) v
         __syncthreads();
         const uint64_t w = s_word;
         int64_t rv = 0, rf = 0;
+        uint32_t rfl = 0;
         if (__builtin_amdgcn_readfirstlane((int)diso_scan::published(w, tag))) {  // published meanwhile
-            rv = wave == 0 ? diso_scan::count_v(w) : 0;  // count it once
-            rf = wave == 0 ? diso_scan::count_f(w) : 0;
+            rv = diso_scan::count_v(w);
+            rf = diso_scan::count_f(w);
+            rfl = diso_scan::flags_of(w);
+            if (wave != 0) rv = rf = 0, rfl = 0;  // count it once
         } else {
             const int64_t jl = (int64_t)j * nseg / nblocks, jh = ((int64_t)j + 1) * nseg / nblocks;
-            sum_counts(seg, jl + (jh - jl) * wave / kCountWaves, jl + (jh - jl) * (wave + 1) / kCountWaves, lane, rv,
-                       rf);
+            count_range<S, ALGO, BATCH, false>(src, grids, seg, info, s_case, jl + (jh - jl) * wave / kCountWaves,
+                                               jl + (jh - jl) * (wave + 1) / kCountWaves, lane, rv, rf, rfl);
+            rfl = wave_or(rfl);
         }
         __syncthreads();  // s_v / s_f reuse
         if (lane == 0) {  // rv, rf are wave-uniform sums
             s_v[wave] = rv;
             s_f[wave] = rf;
+            s_fl[wave] = rfl;
         }
         __syncthreads();
 #pragma unroll
         for (int w2 = 0; w2 < kCountWaves; ++w2) {
             ev += s_v[w2];
             ef += s_f[w2];
+            efl |= s_fl[w2];
         }
     }
     if (last && tid == 0) {
         SegBase *tot = seg + nseg;
-        const uint32_t sfl = __hip_atomic_load(&hdr->sample_flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         tot[0].v = ev + av;
         tot[0].f = ef + af;
-        tot[1].v = (int64_t)(efl | sfl);
-        tot[1].f = src.algo;
-        __hip_atomic_store(&hdr->sample_flags, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // for the next call
+        tot[1].v = (int64_t)(efl | afl);
+        tot[1].f = ALGO;
         __hip_atomic_store(&hdr->dirty[parity], (uint32_t)nblocks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&hdr->dirty[parity ^ 1u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&hdr->epoch, epoch + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -972,13 +817,15 @@ __global__ __launch_bounds__(kCountBlock, kCountBlocksPerCU * kCountWaves / 4) v
     }
 
     // 4. Segment bases: block-wide exclusive scan of this block's segment counts.
-    __syncthreads();  // s_v / s_f reuse
+    __syncthreads();  // s_v / s_f reuse; phase-1 counts complete
     for (int64_t base = lo; base < hi; base += kCountBlock) {
         const int64_t s = base + tid;
         int64_t cv = 0, cf = 0;
         if (s < hi) {
-            cv = seg[s].v;
-            cf = seg[s].f;
+            cv = (int64_t)__hip_atomic_load(reinterpret_cast<uint64_t *>(&seg[s].v), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+            cf = (int64_t)__hip_atomic_load(reinterpret_cast<uint64_t *>(&seg[s].f), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
         }
         const int64_t iv = wave_incl64(cv), jf = wave_incl64(cf);
         if (lane == kWave - 1) {
@@ -2259,17 +2106,14 @@ int check_hip(hipError_t e, const char *what) {
     return fail(DISO_E_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-// Launch parameters (diso_set_param): emission / backward run length, scan blocks per CU,
-// recount spins, classification brick shape.
+// Launch parameters (diso_set_param): emission / backward run length, count blocks per CU,
+// recount spins.
 struct Params {
     int run = 32;
     int count_per_cu = kCountBlocksPerCU;
     int recount_spins = 8192;
-    Brick brick = {8, 8, 8};
 };
 Params g_params;
-constexpr int kMaxBrickRows = 32, kMaxBrickChunks = 32;
-constexpr int64_t kMaxBrickLds = 64 * 1024;  // staged words of one brick, bytes
 
 bool make_geo(int64_t X, int64_t Y, int64_t Z, int flags, Geo *g) {
     if (X < 1 || Y < 1 || Z < 1) return false;
@@ -2342,43 +2186,26 @@ In<S> make_in(const void *sdf, const void *deform, double iso, const Geo &g) {
     return in;
 }
 
-// Bricks of a grid and the staged-word bytes a classification block needs (the largest halo:
-// DMC's, rows -1..+2; words: the brick's chunks, the next and the previous one).
-int64_t host_bricks(const Geo &g, const Brick &br) {
-    return (int64_t)((g.g0 + br.tx - 1) / br.tx) * ((g.g1 + br.ty - 1) / br.ty) * ((g.nchunk + br.kz - 1) / br.kz);
-}
-int64_t brick_lds_bytes(const Brick &br, int nchunk_max) {
-    const int words = br.kz + 2 < nchunk_max ? br.kz + 2 : nchunk_max;
-    return 2 * (int64_t)(br.tx + 3) * (br.ty + 3) * words * (int64_t)sizeof(uint64_t);
-}
-
-// The scan of pass 1 over `nseg` segment counts in `seg` (bases, totals and, for a single grid,
-// its sample flags from the control block's header).
-int launch_scan(int algo, int64_t nseg, SegBase *seg, void *ctl, hipStream_t st) {
-    ScanSrc src{nseg, (uint32_t)g_params.recount_spins, algo};
-    const int64_t nb = count_blocks(nseg, count_resident_per_cu<scan_kernel>());
-    hipLaunchKernelGGL(scan_kernel, dim3((unsigned)nb), dim3(kCountBlock), 0, st, src, seg, (uint8_t *)ctl, nb);
-    return check_hip(hipGetLastError(), "scan kernel launch");
-}
-
 template <typename S>
 int launch_count(int algo, const void *sdf, double iso, const Geo &g, void *seg, void *ctl,
                  int64_t ctl_bytes, hipStream_t st) {
-    (void)ctl_bytes;
-    ClassSrc<S> src{};
+    CountSrc<S> src{};
     src.in = make_in<S>(sdf, nullptr, iso, g);
-    src.algo = algo;
-    src.flags = &reinterpret_cast<CtlHeader *>(ctl)->sample_flags;
-    src.br = g_params.brick;
+    src.nseg = g.nseg;
+    src.recount_spins = (uint32_t)g_params.recount_spins;
+    (void)ctl_bytes;
     SegBase *sb = (SegBase *)seg;
     uint16_t *info = reinterpret_cast<uint16_t *>(sb + g.nseg + 2);
-    const int64_t nb = host_bricks(g, src.br);
-    hipLaunchKernelGGL((classify_kernel<S, false>), dim3((unsigned)nb), dim3(kClassBlock),
-                       (unsigned)brick_lds_bytes(src.br, g.nchunk), st, src, (const BatchGrid<S> *)nullptr, sb, info,
-                       nb);
-    const int rc = check_hip(hipGetLastError(), "classification kernel launch");
-    if (rc) return rc;
-    return launch_scan(algo, g.nseg, sb, ctl, st);
+    if (algo == DISO_MC) {
+        const int64_t nb = count_blocks(g.nseg, count_resident_per_cu<count_kernel<S, DISO_MC, false>>());
+        hipLaunchKernelGGL((count_kernel<S, DISO_MC, false>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, src,
+                           (const BatchGrid<S> *)nullptr, sb, info, (uint8_t *)ctl, nb);
+    } else {
+        const int64_t nb = count_blocks(g.nseg, count_resident_per_cu<count_kernel<S, DISO_DMC, false>>());
+        hipLaunchKernelGGL((count_kernel<S, DISO_DMC, false>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, src,
+                           (const BatchGrid<S> *)nullptr, sb, info, (uint8_t *)ctl, nb);
+    }
+    return check_hip(hipGetLastError(), "count kernel launch");
 }
 
 int64_t row_blocks(const Geo &g) { return (g.nseg + g_params.run - 1) / g_params.run; }
@@ -2706,35 +2533,21 @@ BatchPtrs<S> batch_ptrs(void *ws, const BatchLayout &L) {
 template <typename S>
 int batch_count(diso_grid_t *grids, const BatchLayout &L, void *ws, void *ctl, hipStream_t st) {
     const std::vector<BatchGrid<S>> d = batch_desc<S>(grids, L);
-    // one launch group: every grid, with its bricks
-    Group gr;
-    const Brick br = g_params.brick;
-    int nchunk_max = 1;
-    for (int i = 0; i < L.n; ++i) {
-        gr.add(i, host_bricks(L.geo[i], br));
-        if (L.geo[i].nchunk > nchunk_max) nchunk_max = L.geo[i].nchunk;
-    }
-    int rc = upload_batch<S>(ws, L, d, &gr, 1, st);
+    int rc = upload_batch<S>(ws, L, d, nullptr, 0, st);
     if (rc) return rc;
     const BatchPtrs<S> P = batch_ptrs<S>(ws, L);
-    uint8_t *w = (uint8_t *)ws;
-    uint32_t *gflags = (uint32_t *)((uint8_t *)ctl + align256(ctl_bytes_for(Geo{})));
-    ClassSrc<S> src{};
-    src.flags = gflags;
-    src.br = br;
-    src.gl = P.gl(L, 0, w);
-    src.rf = P.rf(L, 0, w);
-    src.m = gr.m();
-    const int64_t nb = gr.blocks();
-    hipLaunchKernelGGL((classify_kernel<S, true>), dim3((unsigned)nb), dim3(kClassBlock),
-                       (unsigned)brick_lds_bytes(br, nchunk_max), st, src, (const BatchGrid<S> *)P.desc, P.seg, P.info,
-                       nb);
-    rc = check_hip(hipGetLastError(), "batch classification kernel launch");
-    if (rc) return rc;
-    rc = launch_scan(DISO_MC, L.nseg, P.seg, ctl, st);
+    CountSrc<S> src{};
+    src.gflags = (uint32_t *)((uint8_t *)ctl + align256(ctl_bytes_for(Geo{})));
+    src.ngrids = L.n;
+    src.nseg = L.nseg;
+    src.recount_spins = (uint32_t)g_params.recount_spins;
+    const int64_t nb = count_blocks(L.nseg, count_resident_per_cu<count_kernel<S, DISO_MC, true>>());
+    hipLaunchKernelGGL((count_kernel<S, DISO_MC, true>), dim3((unsigned)nb), dim3(kCountBlock), 0, st, src,
+                       (const BatchGrid<S> *)P.desc, P.seg, P.info, (uint8_t *)ctl, nb);
+    rc = check_hip(hipGetLastError(), "batch count kernel launch");
     if (rc) return rc;
     hipLaunchKernelGGL((batch_totals_kernel<S>), dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, st, P.desc, L.n,
-                       L.nseg, P.seg, gflags, P.totals);
+                       L.nseg, P.seg, src.gflags, P.totals);
     rc = check_hip(hipGetLastError(), "batch totals kernel launch");
     if (rc) return rc;
     Staging &stg = staging();
@@ -2995,19 +2808,6 @@ int diso_set_param(int which, int value) {
                 return fail(DISO_E_INVALID, "recount spins must be in [0, %u]", kSpinLimit);
             g_params.recount_spins = value;
             return DISO_OK;
-        case DISO_PARAM_BRICK_X:
-        case DISO_PARAM_BRICK_Y:
-        case DISO_PARAM_BRICK_Z: {
-            const int lim = which == DISO_PARAM_BRICK_Z ? kMaxBrickChunks : kMaxBrickRows;
-            if (value < 1 || value > lim) return fail(DISO_E_INVALID, "brick extent must be in [1, %d]", lim);
-            Brick br = g_params.brick;
-            (which == DISO_PARAM_BRICK_X ? br.tx : which == DISO_PARAM_BRICK_Y ? br.ty : br.kz) = value;
-            if (brick_lds_bytes(br, kMaxBrickChunks + 2) > kMaxBrickLds)
-                return fail(DISO_E_INVALID, "brick %d x %d x %d stages more than %lld bytes", br.tx, br.ty, br.kz,
-                            (long long)kMaxBrickLds);
-            g_params.brick = br;
-            return DISO_OK;
-        }
         default:
             return fail(DISO_E_INVALID, "unknown parameter %d", which);
     }
@@ -3018,9 +2818,6 @@ int diso_get_param(int which) {
         case DISO_PARAM_RUN: return g_params.run;
         case DISO_PARAM_COUNT_BLOCKS_PER_CU: return g_params.count_per_cu;
         case DISO_PARAM_RECOUNT_SPINS: return g_params.recount_spins;
-        case DISO_PARAM_BRICK_X: return g_params.brick.tx;
-        case DISO_PARAM_BRICK_Y: return g_params.brick.ty;
-        case DISO_PARAM_BRICK_Z: return g_params.brick.kz;
         default: return fail(DISO_E_INVALID, "unknown parameter %d", which);
     }
 }

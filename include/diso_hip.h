@@ -197,21 +197,14 @@ int diso_batch_backward(int dtype, const diso_grid_t *grids, int n, void *seg, i
 /* Launch parameters, process-wide (set them before launching work; the autotuner in
  * diso_amd/autotune.py picks them per machine). Results do not depend on them.
  *  DISO_PARAM_RUN: segments per emission / backward block, 1..64 (default 32).
- *  DISO_PARAM_COUNT_BLOCKS_PER_CU: resident blocks per CU of the count pass's scan, 1..4
- *    (default 4).
- *  DISO_PARAM_RECOUNT_SPINS: reads of a predecessor's unpublished aggregate before a scan block
- *    sums that range itself, 0..2^20 (default 8192; 0 sums every predecessor not already
- *    published, which tests use to exercise that path).
- *  DISO_PARAM_BRICK_X / _Y / _Z: the classification brick, rows along x and y (1..32) and
- *    64-position chunks along z (1..32), default 8 x 8 x 8; refused when its staged words would
- *    exceed 64 KB of LDS. Results do not depend on any parameter. */
+ *  DISO_PARAM_COUNT_BLOCKS_PER_CU: resident blocks per CU of the count pass, 1..4 (default 4).
+ *  DISO_PARAM_RECOUNT_SPINS: reads of a predecessor's unpublished aggregate before a count block
+ *    counts that range itself, 0..2^20 (default 8192; 0 counts every predecessor not already
+ *    published, which tests use to exercise that path). */
 enum {
     DISO_PARAM_RUN = 0,
     DISO_PARAM_COUNT_BLOCKS_PER_CU = 1,
-    DISO_PARAM_RECOUNT_SPINS = 2,
-    DISO_PARAM_BRICK_X = 3,
-    DISO_PARAM_BRICK_Y = 4,
-    DISO_PARAM_BRICK_Z = 5
+    DISO_PARAM_RECOUNT_SPINS = 2
 };
 int diso_set_param(int which, int value);
 int diso_get_param(int which);

@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_LIB_PATH = os.environ.get("DISO_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")  # override: a sanitizer build
 _lib = None
 
 MC, DMC = 0, 1
