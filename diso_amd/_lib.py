@@ -46,7 +46,7 @@ EXPORTS = [
     "diso_seg_bytes", "diso_ctl_bytes", "diso_scratch_bytes", "diso_count", "diso_emit", "diso_backward",
     "diso_read_totals", "diso_last_error", "diso_version", "diso_split_scratch_bytes", "diso_split",
     "diso_batch_seg_bytes", "diso_batch_ctl_bytes", "diso_batch_count", "diso_batch_emit", "diso_batch_scratch_bytes",
-    "diso_batch_backward", "diso_set_param", "diso_get_param", "diso_totals_offset",
+    "diso_batch_backward", "diso_set_param", "diso_get_param", "diso_totals_offset", "diso_count_noalloc",
 ]
 
 
@@ -91,6 +91,8 @@ def lib():
         L.diso_count.restype = i32
         L.diso_count.argtypes = [i32, i32, vp, i64, i64, i64, dbl, i32, vp, i64, vp, i64,
                                  ctypes.POINTER(Totals), vp]
+        L.diso_count_noalloc.restype = i32
+        L.diso_count_noalloc.argtypes = [i32, i32, vp, i64, i64, i64, dbl, i32, vp, i64, ctypes.POINTER(Totals), vp]
         L.diso_scratch_bytes.restype = i64
         L.diso_scratch_bytes.argtypes = [i32, i32, i64]
         L.diso_emit.restype = i32

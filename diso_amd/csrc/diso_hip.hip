@@ -37,6 +37,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "diso_hip.h"
@@ -2334,9 +2335,11 @@ SegBase *pinned_totals() {
     return p;
 }
 
-int read_totals(const Geo &g, const void *seg, diso_totals_t *t, hipStream_t st) {
+// `pinned`: stage the copy in this thread's pinned buffer (allocated on first use); otherwise in
+// a stack buffer (diso_count_noalloc allocates nothing).
+int read_totals(const Geo &g, const void *seg, diso_totals_t *t, hipStream_t st, bool pinned = true) {
     SegBase local[2];
-    SegBase *tot = pinned_totals();
+    SegBase *tot = pinned ? pinned_totals() : nullptr;
     if (!tot) tot = local;
     int rc = check_hip(hipMemcpyAsync(tot, (const SegBase *)seg + g.nseg, 2 * sizeof(SegBase), hipMemcpyDeviceToHost,
                                       st),
@@ -2656,6 +2659,52 @@ int batch_backward(const diso_grid_t *grids, const BatchLayout &L, void *ws, voi
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
+// Counting without allocation (diso_count_noalloc): the single-pass scan's control block lives
+// in this file-scope device array instead of a caller workspace. Each device holds its own copy
+// (at its own address, from hipGetSymbolAddress) until the process ends. It is cleared once per
+// device by a fill kernel on the first call's stream; after that the scan keeps it consistent
+// from call to call (epoch-tagged words, each call clears the buffer the next one uses), so no
+// per-call clearing is needed. Every call on a device shares the one array: calls that overlap
+// in time on one device race.
+constexpr int64_t kNoallocCtlBytes = kCtlHeader + 2 * kMaxCountBlocks * (int64_t)sizeof(uint64_t);
+static_assert(kNoallocCtlBytes % 8 == 0, "control block is cleared in 8-byte words");
+static __device__ __align__(256) uint8_t scratch[kNoallocCtlBytes];
+
+static __global__ void clear_words_kernel(uint64_t *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+namespace {
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64];  // device address of `scratch`, per device ordinal
+
+// The current device's copy of `scratch` (looked up, and cleared on `st`, on the device's first
+// call). Host code uses only this address, never the array's name.
+int noalloc_ctl(hipStream_t st, void **out) {
+    int dev = -1;
+    int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= 64) return fail(DISO_E_INVALID, "device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        if (!p) return fail(DISO_E_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        hipLaunchKernelGGL(clear_words_kernel, dim3(16), dim3(256), 0, st, (uint64_t *)p, kNoallocCtlBytes / 8);
+        rc = check_hip(hipGetLastError(), "control block clear launch");
+        if (rc) return rc;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return DISO_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
 // C ABI
 
 extern "C" {
@@ -2702,6 +2751,35 @@ int diso_count(int algo, int dtype, const void *sdf, int64_t X, int64_t Y, int64
     if (rc) return rc;
     if (totals_host) {
         rc = read_totals(g, seg, totals_host, st);
+        if (rc) return rc;
+        if (totals_host->flags & DISO_TOTAL_SCAN_TIMEOUT)
+            return fail(DISO_E_TIMEOUT, "single-pass scan did not complete");
+    }
+    return DISO_OK;
+}
+
+int diso_count_noalloc(int algo, int dtype, const void *sdf, int64_t X, int64_t Y, int64_t Z, double iso,
+                       int flags, void *seg, int64_t seg_bytes, diso_totals_t *totals_host, void *stream) {
+    g_err[0] = 0;
+    Geo g;
+    int rc = check_common(algo, dtype, X, Y, Z, flags, &g);
+    if (rc) return rc;
+    if (!sdf || !seg) return fail(DISO_E_INVALID, "null pointer argument");
+    if (seg_bytes < seg_bytes_for(g))
+        return fail(DISO_E_CAPACITY, "seg workspace too small: %lld < %lld bytes", (long long)seg_bytes,
+                    (long long)seg_bytes_for(g));
+    if (ctl_bytes_for(g) > kNoallocCtlBytes)
+        return fail(DISO_E_CAPACITY, "grid needs a %lld-byte control block, more than the %lld bytes of the "
+                    "library's own", (long long)ctl_bytes_for(g), (long long)kNoallocCtlBytes);
+    hipStream_t st = (hipStream_t)stream;
+    void *ctl = nullptr;
+    rc = noalloc_ctl(st, &ctl);
+    if (rc) return rc;
+    rc = dtype == DISO_F32 ? launch_count<float>(algo, sdf, iso, g, seg, ctl, kNoallocCtlBytes, st)
+                           : launch_count<double>(algo, sdf, iso, g, seg, ctl, kNoallocCtlBytes, st);
+    if (rc) return rc;
+    if (totals_host) {
+        rc = read_totals(g, seg, totals_host, st, false);
         if (rc) return rc;
         if (totals_host->flags & DISO_TOTAL_SCAN_TIMEOUT)
             return fail(DISO_E_TIMEOUT, "single-pass scan did not complete");

@@ -108,6 +108,20 @@ int diso_count(int algo, int dtype, const void *sdf, int64_t X, int64_t Y, int64
                int flags, void *seg, int64_t seg_bytes, void *ctl, int64_t ctl_bytes,
                diso_totals_t *totals_host, void *stream);
 
+/* diso_count without a caller workspace for the scan's control block and without any
+ * allocation: no counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * control block is the library's own 128 KiB array per device (131,136 bytes; it serves every
+ * grid that diso_count accepts, and a grid that needed more would be rejected with
+ * DISO_E_CAPACITY). Every call on a device shares it, so calls that overlap in time on one device
+ * race (as calls sharing one `ctl` do); calls on different devices do not. Results equal
+ * diso_count's. With totals_host set, the totals are copied through a stack buffer (no pinned
+ * staging is allocated) and the stream is synchronised; with totals_host NULL the call only
+ * enqueues work. */
+int diso_count_noalloc(int algo, int dtype, const void *sdf, int64_t X, int64_t Y, int64_t Z,
+                       double iso, int flags, void *seg, int64_t seg_bytes,
+                       diso_totals_t *totals_host, void *stream);
+
 /* Size in bytes of the scratch buffer diso_backward needs for a counted grid (n_crossing from
  * diso_totals_t): none for MC; for DMC, the per-edge upstream gradients, n_crossing * 3
  * scalars. */
