@@ -15,9 +15,14 @@ samples it must touch plus the vertices, faces and dense gradients it must write
 `cpu_baseline` runs the OpenMP CPU oracle (the reference's per-cell algorithm) on the same
 workload on the host cores.
 
-Further legs, keys of the same line: `batched` (64 grids, N in 32..256) and `batched_small` (512
-grids, N in 16..64), each as one batched call per pass against a loop of per-grid calls;
-`serving` (continuous batching under Poisson arrivals); `learn` (graph-captured 512^3 fit).
+`roofline.step_frac` is the whole step's algorithmic bytes (both algorithms' count, emit and
+backward) over `ms_per_step`, against HBM peak.
+
+Further legs, keys of the same line: `round_cube_64` (a 64^3 rounded box, one call at a time, for
+context beside the reference's README table); `batched` (64 grids, N in 32..256) and
+`batched_small` (512 grids, N in 16..64), each as one batched call per pass against a loop of
+per-grid calls; `serving` (continuous batching under Poisson arrivals); `learn` (graph-captured
+512^3 fit).
 """
 import argparse
 import ctypes
@@ -44,7 +49,8 @@ def parse():
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--kernel-reps", type=int, default=10)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-reps", type=int, default=3)
+    p.add_argument("--cpu-reps", type=int, default=5)
+    p.add_argument("--cube-reps", type=int, default=50, help="calls per module of the 64^3 round-cube leg (0: skip)")
     p.add_argument("--copy-mib", type=int, default=256)
     p.add_argument("--batch-count", type=int, default=64, help="grids of the uneven batch (0: skip it)")
     p.add_argument("--batch-small-count", type=int, default=512,
@@ -400,8 +406,55 @@ def cpu_baseline(sdf_np, deform_np, n, seed, reps):
         "cores": threads,
         "kind": "port",
         "sample": f"full workload: MC+DMC fwd+bwd on the {n}^3 grid, median of {reps} runs after 1 warm-up "
-                  f"({med:.2f} s per step; OpenMP oracle, {threads} threads)",
+                  f"({med:.2f} s per step; OpenMP oracle built -O3, {threads} threads)",
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
+        "runs_s": [round(t, 3) for t in times],
     }
+
+
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def round_cube(dev, reps, n=64):
+    """For context beside the reference's README table (README.md:79-94, RTX 4090, grid size not
+    stated): a rounded box (half-extent 0.35, radius 0.1) at n^3, no deform, one DiffMC and one
+    DiffDMC call at a time through the public API, forward alone and forward+backward, end to end
+    (host time per call, synchronised); p50 over `reps` calls."""
+    import torch
+
+    import diso_amd
+    from diso_amd import workloads
+
+    sdf0 = torch.tensor(workloads.round_box(n).astype(np.float32), device=dev)
+    out = {"grid": n, "workload": f"rounded box half-extent 0.35 radius 0.1 at {n}^3, no deform, end-to-end per call"}
+    for name, mod, kw in (("mc", diso_amd.DiffMC(torch.float32), {}),
+                          ("dmc", diso_amd.DiffDMC(torch.float32), {})):
+        res = {}
+        for mode in ("fwd", "fwd_bwd"):
+            ts = []
+            for r in range(reps + 3):
+                sdf = sdf0.detach().requires_grad_(mode == "fwd_bwd")
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                v, f = mod(sdf, None, 0.0, **kw)
+                if mode == "fwd_bwd":
+                    v.sum().backward()
+                torch.cuda.synchronize()
+                if r >= 3:
+                    ts.append(time.perf_counter() - t0)
+            res[mode + "_p50_ms"] = round(float(np.percentile(ts, 50)) * 1e3, 3)
+            res[mode + "_p99_ms"] = round(float(np.percentile(ts, 99)) * 1e3, 3)
+        res["verts"], res["faces"] = int(v.shape[0]), int(f.shape[0])
+        out[name] = res
+    return out
 
 
 def main():
@@ -413,7 +466,7 @@ def main():
     import torch
 
     import diso_amd
-    from diso_amd import _lib, workloads
+    from diso_amd import _lib, autotune, workloads
 
     dev = torch.device("cuda:0")
     n = args.n
@@ -473,6 +526,7 @@ def main():
     timed = {k: v for k, v in kern.items() if "ms" in v}
     dom = max(timed, key=lambda k: timed[k]["ms"])
     d = timed[dom]
+    step_bytes = sum(v["bytes"] for v in timed.values())  # both algorithms' count + emit + backward
     copy = copy_calibration(dev, args.copy_mib, args.kernel_reps)
 
     out = {
@@ -492,13 +546,17 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain DiffMC + DiffDMC(quads) fwd+bwd, rounded box + U(-0.05,0.05) noise, "
                                f"deform U(-0.3,0.3), N={n}", "grid": n, "grids_per_step": 2,
-                   "mode": "plain per-grid API", "touched_vertices": tv},
+                   "mode": "plain per-grid API", "touched_vertices": tv, "launch_params": autotune.current()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(d["GBs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, n),
-                     "traffic_source": f"profiles/hbm_traffic_bench_n{n}.txt"},
+                     "traffic_source": f"profiles/hbm_traffic_bench_n{n}.txt",
+                     "step_bytes": int(step_bytes),
+                     "step_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": kern,
         "copy_calibration": copy,
     }
+    if args.cube_reps > 0:
+        out["round_cube_64"] = round_cube(dev, args.cube_reps)
     if args.batch_count > 0:
         del sets, mods
         torch.cuda.empty_cache()

@@ -83,6 +83,13 @@ struct V3 {
     S x, y, z;
 };
 
+// Stores of outputs that no later kernel reads (vertices, faces, dense gradients): non-temporal,
+// so these streams do not evict from L2 the inputs that the gathers re-read.
+template <typename T>
+__device__ __forceinline__ void put(T *p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+
 template <typename S>
 __device__ __forceinline__ bool in_data(const In<S> &in, int x, int y, int z) {
     return (unsigned)(x - in.geo.off) < (unsigned)in.geo.n0 &&
@@ -929,9 +936,9 @@ __device__ __forceinline__ V3<S> epilogue(V3<S> p, int flags, const Norm<S> &nm)
 
 __device__ __forceinline__ void store_index(void *__restrict__ faces, int64_t i, int64_t v, int flags) {
     if (flags & DISO_FLAG_INDEX64)
-        reinterpret_cast<int64_t *>(faces)[i] = v;
+        put(reinterpret_cast<int64_t *>(faces) + i, (int64_t)v);
     else
-        reinterpret_cast<int32_t *>(faces)[i] = (int32_t)v;
+        put(reinterpret_cast<int32_t *>(faces) + i, (int32_t)v);
 }
 
 // Quad q = base + rel[0..3], as four int64 or int32 indices; `vec`: `faces` is 16-byte aligned,
@@ -943,20 +950,20 @@ __device__ __forceinline__ void store_quad(void *__restrict__ faces, int64_t q, 
     if (flags & DISO_FLAG_INDEX64) {
         int64_t *f = reinterpret_cast<int64_t *>(faces) + 4 * q;
         if (vec) {
-            reinterpret_cast<i64x2 *>(f)[0] = i64x2{base + rel[0], base + rel[1]};
-            reinterpret_cast<i64x2 *>(f)[1] = i64x2{base + rel[2], base + rel[3]};
+            put(reinterpret_cast<i64x2 *>(f), i64x2{base + rel[0], base + rel[1]});
+            put(reinterpret_cast<i64x2 *>(f) + 1, i64x2{base + rel[2], base + rel[3]});
         } else {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) f[j] = base + rel[j];
+            for (int j = 0; j < 4; ++j) put(f + j, (int64_t)(base + rel[j]));
         }
     } else {
         int32_t *f = reinterpret_cast<int32_t *>(faces) + 4 * q;
         if (vec) {
-            *reinterpret_cast<i32x4 *>(f) = i32x4{(int32_t)(base + rel[0]), (int32_t)(base + rel[1]),
-                                                  (int32_t)(base + rel[2]), (int32_t)(base + rel[3])};
+            put(reinterpret_cast<i32x4 *>(f), i32x4{(int32_t)(base + rel[0]), (int32_t)(base + rel[1]),
+                                                  (int32_t)(base + rel[2]), (int32_t)(base + rel[3])});
         } else {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) f[j] = (int32_t)(base + rel[j]);
+            for (int j = 0; j < 4; ++j) put(f + j, (int32_t)(base + rel[j]));
         }
     }
 }
@@ -969,9 +976,9 @@ __device__ __forceinline__ int64_t load_index(const void *__restrict__ faces, in
 template <typename S>
 __device__ __forceinline__ void store_vertex(S *__restrict__ verts, int64_t i, int64_t cap, V3<S> p) {
     if (i < cap) {
-        verts[3 * i + 0] = p.x;
-        verts[3 * i + 1] = p.y;
-        verts[3 * i + 2] = p.z;
+        put(verts + 3 * i + 0, p.x);
+        put(verts + 3 * i + 1, p.y);
+        put(verts + 3 * i + 2, p.z);
     }
 }
 
@@ -1392,7 +1399,7 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
         }
         const S r = rcp_small<S>(num);
         store_vertex(verts, pbase + k, out.vcap, epilogue(V3<S>{sx * r, sy * r, sz * r}, out.flags, nm));
-        if (pbase + k < out.vcap) vinfo[pbase + k] = (uint8_t)(num | (pk << 3));
+        if (pbase + k < out.vcap) put(vinfo + pbase + k, (uint8_t)(num | (pk << 3)));
     }
     const bool vec = ((uintptr_t)faces & 15) == 0;  // a quad's four indices in 16-byte stores
     for (int k = lane; k < tq; k += kWave) {
@@ -1540,11 +1547,11 @@ __device__ __forceinline__ void store_grad(const In<S> &in, S *__restrict__ adj_
             adj_deform[3 * i + 2] += acc.dz;
         }
     } else {
-        adj_sdf[i] = acc.g;
+        put(adj_sdf + i, acc.g);
         if (adj_deform) {
-            adj_deform[3 * i + 0] = acc.dx;
-            adj_deform[3 * i + 1] = acc.dy;
-            adj_deform[3 * i + 2] = acc.dz;
+            put(adj_deform + 3 * i + 0, acc.dx);
+            put(adj_deform + 3 * i + 1, acc.dy);
+            put(adj_deform + 3 * i + 2, acc.dz);
         }
     }
 }
@@ -2076,13 +2083,9 @@ __global__ __launch_bounds__(kSplitBlock) void split_emit_kernel(const void *__r
                           c2 = load_index(quads, 4 * q + 2, flags), c3 = load_index(quads, 4 * q + 3, flags);
             // rows 2r, 2r+1 of the first half, or of the second half at rank q - r1 among the others
             int64_t *t = tris + 6 * (f ? r1 : n1 + (q - r1));
-            if (f) {
-                t[0] = a, t[1] = c1, t[2] = c3;
-                t[3] = c1, t[4] = c2, t[5] = c3;
-            } else {
-                t[0] = a, t[1] = c1, t[2] = c2;
-                t[3] = a, t[4] = c2, t[5] = c3;
-            }
+            const int64_t v[6] = {a, c1, f ? c3 : c2, f ? c1 : a, c2, c3};
+#pragma unroll
+            for (int j = 0; j < 6; ++j) put(t + j, v[j]);
         }
         run += blk;
     }
