@@ -18,8 +18,13 @@ workload on the host cores.
 `roofline.step_frac` is the whole step's algorithmic bytes (both algorithms' count, emit and
 backward) over `ms_per_step`, against HBM peak.
 
-Further legs, keys of the same line: `round_cube_64` (a 64^3 rounded box, one call at a time, for
-context beside the reference's README table); `batched` (64 grids, N in 32..256) and
+`config.launch_params` names the launch parameters in force (diso_amd/tuning.json for this
+device when the autotuner saved one, else the library defaults), the size bucket of the headline
+grid and the table of all buckets.
+
+Further legs, keys of the same line: `batched_step` (the headline step as one batched call over
+its two extractions, against the plain step); `round_cube_64` (a 64^3 rounded box, one call at a
+time, for context beside the reference's README table); `batched` (64 grids, N in 32..256) and
 `batched_small` (512 grids, N in 16..64), each as one batched call per pass against a loop of
 per-grid calls; `serving` (continuous batching under Poisson arrivals); `learn` (graph-captured
 512^3 fit).
@@ -243,6 +248,52 @@ def batched_vs_plain(dev, count, seed, reps, sizes=(32, 48, 64, 96, 128, 192, 25
         "plain_loop_ms": round(res["plain_loop"] * 1e3, 3),
         "speedup": round(res["plain_loop"] / res["batched"], 3),
     }
+
+
+def batched_step(sets, ups, steps, warmup):
+    """The headline step (DiffMC + DiffDMC(quads) fwd+bwd on one grid) as one BatchExtractor call
+    over the two extractions, against the plain per-module step, alternating the two input sets
+    as the headline loop does; wall time over `steps` after `warmup`."""
+    import torch
+
+    import diso_amd
+
+    n = sets[0][0].shape[0]
+    mods = {"mc": diso_amd.DiffMC(torch.float32), "dmc": diso_amd.DiffDMC(torch.float32)}
+    bex = diso_amd.BatchExtractor(torch.float32)
+
+    def plain(i):
+        sdf0, def0 = sets[i % 2][0], sets[i % 2][1]
+        for name, mod in mods.items():
+            sdf, deform = sdf0.detach().requires_grad_(True), def0.detach().requires_grad_(True)
+            kw = {"return_quads": True} if name == "dmc" else {}
+            v, _ = mod(sdf, deform, 0.0, **kw)
+            torch.autograd.backward(v, grad_tensors=ups[i % 2][name])
+
+    def batched(i):
+        sdf0, def0 = sets[i % 2][0], sets[i % 2][1]
+        sdfs = [sdf0.detach().requires_grad_(True) for _ in range(2)]
+        defs = [def0.detach().requires_grad_(True) for _ in range(2)]
+        outs = bex(sdfs, ["mc", "dmc"], defs, return_quads=True)
+        torch.autograd.backward([outs[0][0], outs[1][0]], grad_tensors=[ups[i % 2]["mc"], ups[i % 2]["dmc"]])
+
+    res = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        for i in range(warmup):
+            fn(i)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            fn(i)
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / steps
+    vox = 2 * n ** 3
+    return {"workload": f"the headline step (DiffMC + DiffDMC(quads) fwd+bwd, N={n}) as one BatchExtractor call "
+                        "over its two extractions vs the plain per-module calls",
+            "batched_gvoxel_s": round(vox / res["batched"] / 1e9, 3),
+            "plain_gvoxel_s": round(vox / res["plain"] / 1e9, 3),
+            "batched_ms": round(res["batched"] * 1e3, 3), "plain_ms": round(res["plain"] * 1e3, 3),
+            "speedup": round(res["plain"] / res["batched"], 3)}
 
 
 def learnable_fit(dev, n, n_stored, steps):
@@ -546,7 +597,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain DiffMC + DiffDMC(quads) fwd+bwd, rounded box + U(-0.05,0.05) noise, "
                                f"deform U(-0.3,0.3), N={n}", "grid": n, "grids_per_step": 2,
-                   "mode": "plain per-grid API", "touched_vertices": tv, "launch_params": autotune.current()},
+                   "mode": "plain per-grid API", "touched_vertices": tv,
+                   "launch_params": dict(autotune.applied(), bucket=str(autotune.BUCKET_SIZES[
+                       _lib.lib().diso_param_bucket(n, n, n, _lib.FLAG_VIRTUAL_PAD)]), buckets=autotune.current())},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(d["GBs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, n),
                      "traffic_source": f"profiles/hbm_traffic_bench_n{n}.txt",
@@ -555,6 +608,7 @@ def main():
         "kernels": kern,
         "copy_calibration": copy,
     }
+    out["batched_step"] = batched_step(sets, ups, args.steps, args.warmup)
     if args.cube_reps > 0:
         out["round_cube_64"] = round_cube(dev, args.cube_reps)
     if args.batch_count > 0:

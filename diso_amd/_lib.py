@@ -41,12 +41,14 @@ E_TIMEOUT = -4  # DISO_E_TIMEOUT: the single-pass scan did not complete (never e
 PARAM_RUN = 0
 PARAM_COUNT_BLOCKS_PER_CU = 1
 PARAM_RECOUNT_SPINS = 2
+PARAM_BUCKETS = 5  # size buckets of the run length and count blocks per CU (~32^3 .. ~512^3 grids)
 
 EXPORTS = [
     "diso_seg_bytes", "diso_ctl_bytes", "diso_scratch_bytes", "diso_count", "diso_emit", "diso_backward",
     "diso_read_totals", "diso_last_error", "diso_version", "diso_split_scratch_bytes", "diso_split",
     "diso_batch_seg_bytes", "diso_batch_ctl_bytes", "diso_batch_count", "diso_batch_emit", "diso_batch_scratch_bytes",
     "diso_batch_backward", "diso_set_param", "diso_get_param", "diso_totals_offset", "diso_count_noalloc",
+    "diso_set_bucket_param", "diso_get_bucket_param", "diso_param_bucket",
 ]
 
 
@@ -121,6 +123,12 @@ def lib():
         L.diso_set_param.argtypes = [i32, i32]
         L.diso_get_param.restype = i32
         L.diso_get_param.argtypes = [i32]
+        L.diso_set_bucket_param.restype = i32
+        L.diso_set_bucket_param.argtypes = [i32, i32, i32]
+        L.diso_get_bucket_param.restype = i32
+        L.diso_get_bucket_param.argtypes = [i32, i32]
+        L.diso_param_bucket.restype = i32
+        L.diso_param_bucket.argtypes = [i64, i64, i64, i32]
         L.diso_read_totals.restype = i32
         L.diso_read_totals.argtypes = [vp, i64, i64, i64, i64, i32, ctypes.POINTER(Totals), vp]
         L.diso_last_error.restype = ctypes.c_char_p

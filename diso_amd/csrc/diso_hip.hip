@@ -2110,14 +2110,48 @@ int check_hip(hipError_t e, const char *what) {
     return fail(DISO_E_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-// Launch parameters (diso_set_param): emission / backward run length, count blocks per CU,
-// recount spins.
+// Launch parameters (diso_set_param, diso_set_bucket_param): emission / backward run length and
+// count blocks per CU, each per size bucket, and the recount spins. A launch takes the bucket of
+// its segment count (a batched launch: the segments of all its grids); the bucket limits lie
+// between the segment counts of cubic grids of 32, 64, 128, 256 and 512 samples per side with
+// the virtual pad (1,156 / 8,712 / 50,700 / 332,820 / 2,377,764 segments).
+constexpr int kBuckets = DISO_PARAM_BUCKETS;
+constexpr int64_t kBucketLimit[kBuckets - 1] = {3200, 21000, 130000, 900000};
 struct Params {
-    int run = 32;
-    int count_per_cu = kCountBlocksPerCU;
+    int run[kBuckets] = {32, 32, 32, 32, 32};
+    int count_per_cu[kBuckets] = {kCountBlocksPerCU, kCountBlocksPerCU, kCountBlocksPerCU, kCountBlocksPerCU,
+                                  kCountBlocksPerCU};
     int recount_spins = 8192;
 };
 Params g_params;
+
+int bucket_of(int64_t nseg) {
+    int b = 0;
+    while (b < kBuckets - 1 && nseg > kBucketLimit[b]) ++b;
+    return b;
+}
+int run_for(int64_t nseg) { return g_params.run[bucket_of(nseg)]; }
+
+// Validates a per-bucket parameter value; returns its table (nullptr after fail()).
+int *bucket_table(int which, int value) {
+    switch (which) {
+        case DISO_PARAM_RUN:
+            if (value < 1 || value > kMaxRun) {
+                fail(DISO_E_INVALID, "run length must be in [1, %d]", kMaxRun);
+                return nullptr;
+            }
+            return g_params.run;
+        case DISO_PARAM_COUNT_BLOCKS_PER_CU:
+            if (value < 1 || value > kCountBlocksPerCU) {
+                fail(DISO_E_INVALID, "count blocks per CU must be in [1, %d]", kCountBlocksPerCU);
+                return nullptr;
+            }
+            return g_params.count_per_cu;
+        default:
+            fail(DISO_E_INVALID, "parameter %d has no size buckets", which);
+            return nullptr;
+    }
+}
 
 bool make_geo(int64_t X, int64_t Y, int64_t Z, int flags, Geo *g) {
     if (X < 1 || Y < 1 || Z < 1) return false;
@@ -2170,7 +2204,8 @@ int count_resident_per_cu() {
 }
 
 int64_t count_blocks(int64_t nseg, int resident_per_cu) {
-    const int per_cu = g_params.count_per_cu < resident_per_cu ? g_params.count_per_cu : resident_per_cu;
+    const int want = g_params.count_per_cu[bucket_of(nseg)];
+    const int per_cu = want < resident_per_cu ? want : resident_per_cu;
     int64_t nb = (int64_t)cu_count() * per_cu;
     const int64_t min_nb = (nseg + (1ll << 20) - 1) >> 20;  // <= 2^20 segments per block keeps the
     if (nb < min_nb) nb = min_nb;                          // aggregate fields from overflowing
@@ -2212,7 +2247,7 @@ int launch_count(int algo, const void *sdf, double iso, const Geo &g, void *seg,
     return check_hip(hipGetLastError(), "count kernel launch");
 }
 
-int64_t row_blocks(const Geo &g) { return (g.nseg + g_params.run - 1) / g_params.run; }
+int64_t row_blocks(const Geo &g) { return (g.nseg + run_for(g.nseg) - 1) / run_for(g.nseg); }
 
 template <typename S>
 Norm<S> norm_of(const Geo &g) {
@@ -2234,7 +2269,7 @@ int launch_emit(int algo, const void *sdf, const void *deform, double iso, int f
     const SegBase *sb = (const SegBase *)seg;
     const OutCfg out{flags, vcap, fcap};
     const dim3 grid((unsigned)nb), block(kBlock);
-    const int run = g_params.run;
+    const int run = run_for(g.nseg);
     if (algo == DISO_MC) {
         if (deform)
             hipLaunchKernelGGL((emit_mc_kernel<S, true>), grid, block, 0, st, in, sb, (S *)verts, faces, out, nm,
@@ -2263,6 +2298,7 @@ int launch_backward(int algo, const void *sdf, const void *deform, double iso, i
     a.n = nverts;
     a.flags = flags;
     a.nm = norm_of<S>(g);
+    const int run = run_for(g.nseg);
     const int64_t nb = row_blocks(g);
     const SegBase *sb = (const SegBase *)seg;
     S *gs = (S *)adj_sdf, *gd = (S *)(deform ? adj_deform : nullptr);
@@ -2282,13 +2318,13 @@ int launch_backward(int algo, const void *sdf, const void *deform, double iso, i
     const dim3 grid((unsigned)nb), block(kBlock);
     const bool rec = (flags & DISO_FLAG_RECOMPUTE) != 0;
     if (deform && rec)
-        hipLaunchKernelGGL((bwd_mc_kernel<S, true, true>), grid, block, 0, st, in, sb, a, gs, gd, g_params.run, nb);
+        hipLaunchKernelGGL((bwd_mc_kernel<S, true, true>), grid, block, 0, st, in, sb, a, gs, gd, run, nb);
     else if (deform)
-        hipLaunchKernelGGL((bwd_mc_kernel<S, true, false>), grid, block, 0, st, in, sb, a, gs, gd, g_params.run, nb);
+        hipLaunchKernelGGL((bwd_mc_kernel<S, true, false>), grid, block, 0, st, in, sb, a, gs, gd, run, nb);
     else if (rec)
-        hipLaunchKernelGGL((bwd_mc_kernel<S, false, true>), grid, block, 0, st, in, sb, a, gs, gd, g_params.run, nb);
+        hipLaunchKernelGGL((bwd_mc_kernel<S, false, true>), grid, block, 0, st, in, sb, a, gs, gd, run, nb);
     else
-        hipLaunchKernelGGL((bwd_mc_kernel<S, false, false>), grid, block, 0, st, in, sb, a, gs, gd, g_params.run, nb);
+        hipLaunchKernelGGL((bwd_mc_kernel<S, false, false>), grid, block, 0, st, in, sb, a, gs, gd, run, nb);
     return check_hip(hipGetLastError(), "backward kernel launch");
 }
 
@@ -2571,10 +2607,11 @@ int batch_count(diso_grid_t *grids, const BatchLayout &L, void *ws, void *ctl, h
     return DISO_OK;
 }
 
-int64_t runs_of(const Geo &g) { return (g.nseg + g_params.run - 1) / g_params.run; }
+int64_t runs_of(const Geo &g, int run) { return (g.nseg + run - 1) / run; }
 
 template <typename S>
 int batch_emit(const diso_grid_t *grids, const BatchLayout &L, void *ws, hipStream_t st) {
+    const int run = run_for(L.nseg);
     Group gr[4];  // MC, MC + deform, DMC, DMC + deform
     for (int i = 0; i < L.n; ++i) {
         const diso_grid_t &G = grids[i];
@@ -2583,7 +2620,7 @@ int batch_emit(const diso_grid_t *grids, const BatchLayout &L, void *ws, hipStre
             return fail(DISO_E_CAPACITY, "grid %d: output buffers too small", i);
         if (!G.verts || !G.faces || (G.algo == DISO_DMC && !G.vinfo))
             return fail(DISO_E_INVALID, "grid %d: null output buffer", i);
-        gr[2 * (G.algo == DISO_DMC) + (G.deform != nullptr)].add(i, runs_of(L.geo[i]));
+        gr[2 * (G.algo == DISO_DMC) + (G.deform != nullptr)].add(i, runs_of(L.geo[i], run));
     }
     int rc = upload_batch<S>(ws, L, batch_desc<S>(grids, L), gr, 4, st);
     if (rc) return rc;
@@ -2597,10 +2634,10 @@ int batch_emit(const diso_grid_t *grids, const BatchLayout &L, void *ws, hipStre
         const int64_t *rf = P.rf(L, k, w);
         const int m = gr[k].m();
         switch (k) {
-            case 0: hipLaunchKernelGGL((emit_mc_batch_kernel<S, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, g_params.run, nb); break;
-            case 1: hipLaunchKernelGGL((emit_mc_batch_kernel<S, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, g_params.run, nb); break;
-            case 2: hipLaunchKernelGGL((emit_dmc_batch_kernel<S, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, g_params.run, nb); break;
-            default: hipLaunchKernelGGL((emit_dmc_batch_kernel<S, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, g_params.run, nb); break;
+            case 0: hipLaunchKernelGGL((emit_mc_batch_kernel<S, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, run, nb); break;
+            case 1: hipLaunchKernelGGL((emit_mc_batch_kernel<S, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, run, nb); break;
+            case 2: hipLaunchKernelGGL((emit_dmc_batch_kernel<S, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, run, nb); break;
+            default: hipLaunchKernelGGL((emit_dmc_batch_kernel<S, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, run, nb); break;
         }
         rc = check_hip(hipGetLastError(), "batch emit kernel launch");
         if (rc) return rc;
@@ -2617,6 +2654,7 @@ int64_t batch_scratch_bytes(int dtype, const diso_grid_t *grids, int n) {
 
 template <typename S>
 int batch_backward(const diso_grid_t *grids, const BatchLayout &L, void *ws, void *scratch, hipStream_t st) {
+    const int run = run_for(L.nseg);
     Group gr[5];  // 0-3: gather, deform x recompute; 4: DMC per-edge gradients
     for (int i = 0; i < L.n; ++i) {
         const diso_grid_t &G = grids[i];
@@ -2627,7 +2665,7 @@ int batch_backward(const diso_grid_t *grids, const BatchLayout &L, void *ws, voi
             if (!G.faces || !G.vinfo) return fail(DISO_E_INVALID, "grid %d: DMC backward needs quads and vinfo", i);
             gr[4].add(i, (G.totals.n_faces + kBlock - 1) / kBlock);
         }
-        gr[2 * (G.deform != nullptr) + ((G.flags & DISO_FLAG_RECOMPUTE) != 0)].add(i, runs_of(L.geo[i]));
+        gr[2 * (G.deform != nullptr) + ((G.flags & DISO_FLAG_RECOMPUTE) != 0)].add(i, runs_of(L.geo[i], run));
     }
     int rc = upload_batch<S>(ws, L, batch_desc<S>(grids, L), gr, 5, st);
     if (rc) return rc;
@@ -2648,10 +2686,10 @@ int batch_backward(const diso_grid_t *grids, const BatchLayout &L, void *ws, voi
         const int64_t *rf = P.rf(L, k, w);
         const int m = gr[k].m();
         switch (k) {
-            case 0: hipLaunchKernelGGL((bwd_batch_kernel<S, false, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, g_params.run, nb); break;
-            case 1: hipLaunchKernelGGL((bwd_batch_kernel<S, false, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, g_params.run, nb); break;
-            case 2: hipLaunchKernelGGL((bwd_batch_kernel<S, true, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, g_params.run, nb); break;
-            default: hipLaunchKernelGGL((bwd_batch_kernel<S, true, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, g_params.run, nb); break;
+            case 0: hipLaunchKernelGGL((bwd_batch_kernel<S, false, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, run, nb); break;
+            case 1: hipLaunchKernelGGL((bwd_batch_kernel<S, false, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, run, nb); break;
+            case 2: hipLaunchKernelGGL((bwd_batch_kernel<S, true, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, run, nb); break;
+            default: hipLaunchKernelGGL((bwd_batch_kernel<S, true, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, run, nb); break;
         }
         rc = check_hip(hipGetLastError(), "batch backward kernel launch");
         if (rc) return rc;
@@ -2876,14 +2914,12 @@ int diso_set_param(int which, int value) {
     g_err[0] = 0;
     switch (which) {
         case DISO_PARAM_RUN:
-            if (value < 1 || value > kMaxRun) return fail(DISO_E_INVALID, "run length must be in [1, %d]", kMaxRun);
-            g_params.run = value;
+        case DISO_PARAM_COUNT_BLOCKS_PER_CU: {
+            int *t = bucket_table(which, value);
+            if (!t) return DISO_E_INVALID;
+            for (int b = 0; b < kBuckets; ++b) t[b] = value;
             return DISO_OK;
-        case DISO_PARAM_COUNT_BLOCKS_PER_CU:
-            if (value < 1 || value > kCountBlocksPerCU)
-                return fail(DISO_E_INVALID, "count blocks per CU must be in [1, %d]", kCountBlocksPerCU);
-            g_params.count_per_cu = value;
-            return DISO_OK;
+        }
         case DISO_PARAM_RECOUNT_SPINS:
             if (value < 0 || value > (int)kSpinLimit)
                 return fail(DISO_E_INVALID, "recount spins must be in [0, %u]", kSpinLimit);
@@ -2896,11 +2932,35 @@ int diso_set_param(int which, int value) {
 
 int diso_get_param(int which) {
     switch (which) {
-        case DISO_PARAM_RUN: return g_params.run;
-        case DISO_PARAM_COUNT_BLOCKS_PER_CU: return g_params.count_per_cu;
+        case DISO_PARAM_RUN: return g_params.run[kBuckets - 2];
+        case DISO_PARAM_COUNT_BLOCKS_PER_CU: return g_params.count_per_cu[kBuckets - 2];
         case DISO_PARAM_RECOUNT_SPINS: return g_params.recount_spins;
         default: return fail(DISO_E_INVALID, "unknown parameter %d", which);
     }
+}
+
+int diso_set_bucket_param(int which, int bucket, int value) {
+    g_err[0] = 0;
+    if (bucket < 0 || bucket >= kBuckets) return fail(DISO_E_INVALID, "bucket must be in [0, %d]", kBuckets - 1);
+    int *t = bucket_table(which, value);
+    if (!t) return DISO_E_INVALID;
+    t[bucket] = value;
+    return DISO_OK;
+}
+
+int diso_get_bucket_param(int which, int bucket) {
+    if (bucket < 0 || bucket >= kBuckets) return fail(DISO_E_INVALID, "bucket must be in [0, %d]", kBuckets - 1);
+    switch (which) {
+        case DISO_PARAM_RUN: return g_params.run[bucket];
+        case DISO_PARAM_COUNT_BLOCKS_PER_CU: return g_params.count_per_cu[bucket];
+        default: return fail(DISO_E_INVALID, "parameter %d has no size buckets", which);
+    }
+}
+
+int diso_param_bucket(int64_t X, int64_t Y, int64_t Z, int flags) {
+    Geo g;
+    if (!make_geo(X, Y, Z, flags, &g)) return fail(DISO_E_INVALID, "bad grid dims");
+    return bucket_of(g.nseg);
 }
 
 int64_t diso_batch_seg_bytes(const diso_grid_t *grids, int n) {

@@ -49,6 +49,23 @@ def noisy_round_box(n, seed=0, amp=0.05, deform_amp=0.3, dtype=np.float32):
     return sdf.astype(dtype), deform.astype(dtype), rng
 
 
+def noisy_round_box_torch(n, seed=0, amp=0.05, deform_amp=0.3, device="cuda", dtype=None):
+    """noisy_round_box's distribution generated on the device with torch (for sizes where numpy
+    is slow; the values differ from noisy_round_box's). Returns (sdf [N,N,N], deform [N,N,N,3])."""
+    import torch
+
+    dtype = dtype or torch.float32
+    g = torch.Generator(device=device).manual_seed(seed)
+    a = torch.linspace(-0.5, 0.5, n, dtype=torch.float64, device=device)
+    p = torch.stack(torch.meshgrid(a, a, a, indexing="ij"), -1)
+    q = p.abs() - 0.35 + 0.1
+    f = q.clamp_min(0).norm(dim=-1) + q.max(-1)[0].clamp_max(0) - 0.1
+    del p, q
+    f += (torch.rand(f.shape, generator=g, dtype=torch.float64, device=device) * 2 - 1) * amp
+    d = (torch.rand((n, n, n, 3), generator=g, dtype=torch.float64, device=device) * 2 - 1) * deform_amp
+    return f.to(dtype).contiguous(), d.to(dtype).contiguous()
+
+
 def upstream(rng, nverts, dtype=np.float32):
     """Upstream dL/dverts ~ N(0,1) of shape [V,3], drawn after V is known."""
     return rng.standard_normal((nverts, 3)).astype(dtype)

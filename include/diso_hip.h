@@ -209,19 +209,30 @@ int diso_batch_backward(int dtype, const diso_grid_t *grids, int n, void *seg, i
                         void *scratch, int64_t scratch_bytes, void *stream);
 
 /* Launch parameters, process-wide (set them before launching work; the autotuner in
- * diso_amd/autotune.py picks them per machine). Results do not depend on them.
+ * diso_amd/autotune.py picks them per machine and grid size). Results do not depend on them.
+ * No counterpart in the reference, whose block size is a compile-time constant
+ * (src/cumc.cu:93, BLOCK_SIZE 512).
  *  DISO_PARAM_RUN: segments per emission / backward block, 1..64 (default 32).
  *  DISO_PARAM_COUNT_BLOCKS_PER_CU: resident blocks per CU of the count pass, 1..4 (default 4).
  *  DISO_PARAM_RECOUNT_SPINS: reads of a predecessor's unpublished aggregate before a count block
  *    counts that range itself, 0..2^20 (default 8192; 0 counts every predecessor not already
- *    published, which tests use to exercise that path). */
+ *    published, which tests use to exercise that path).
+ * The first two have one value per size bucket. A launch takes the bucket of its segment count
+ * (a batched launch: of all its grids together); buckets 0..4 hold grids of about 32^3, 64^3,
+ * 128^3, 256^3 and 512^3 samples (limits 3,200 / 21,000 / 130,000 / 900,000 segments).
+ * diso_set_param sets every bucket; diso_get_param returns bucket 3's value. */
 enum {
     DISO_PARAM_RUN = 0,
     DISO_PARAM_COUNT_BLOCKS_PER_CU = 1,
     DISO_PARAM_RECOUNT_SPINS = 2
 };
+#define DISO_PARAM_BUCKETS 5
 int diso_set_param(int which, int value);
 int diso_get_param(int which);
+int diso_set_bucket_param(int which, int bucket, int value);
+int diso_get_bucket_param(int which, int bucket);
+/* Bucket of a single-grid launch on an X x Y x Z grid (flags as for diso_count). */
+int diso_param_bucket(int64_t X, int64_t Y, int64_t Z, int flags);
 
 /* Reads the totals of the last diso_count on `seg` (blocking copy on `stream`). */
 int diso_read_totals(const void *seg, int64_t seg_bytes, int64_t X, int64_t Y, int64_t Z,

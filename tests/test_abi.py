@@ -157,9 +157,12 @@ def test_launch_parameters():
         assert L.diso_set_param(_lib.PARAM_RUN, 0) == -1 and L.diso_set_param(_lib.PARAM_RUN, 65) == -1
         assert L.diso_set_param(_lib.PARAM_COUNT_BLOCKS_PER_CU, 5) == -1 and L.diso_set_param(9, 1) == -1
         autotune.apply({"run": 16, "count_blocks_per_cu": 2})
-        assert autotune.current() == {"run": 16, "count_blocks_per_cu": 2}
+        assert autotune.current() == {str(n): {"run": 16, "count_blocks_per_cu": 2} for n in autotune.BUCKET_SIZES}
+        assert L.diso_get_param(_lib.PARAM_RUN) == 16 and L.diso_get_param(_lib.PARAM_COUNT_BLOCKS_PER_CU) == 2
         with pytest.raises(RuntimeError):
             autotune.apply({"run": 100, "count_blocks_per_cu": 2})
+        with pytest.raises(ValueError):
+            autotune.apply({"100": {"run": 8}})
         spins = L.diso_get_param(_lib.PARAM_RECOUNT_SPINS)
         assert spins == 8192
         assert L.diso_set_param(_lib.PARAM_RECOUNT_SPINS, -1) == -1
@@ -168,6 +171,45 @@ def test_launch_parameters():
         assert L.diso_set_param(_lib.PARAM_RECOUNT_SPINS, spins) == 0
     finally:
         autotune.apply(before)
+
+
+def test_bucket_parameters():
+    """Per-bucket values are independent; diso_get_param reads bucket 3; invalid buckets, values
+    and parameters are rejected with the message set."""
+    from diso_amd import autotune
+
+    L = _lib.lib()
+    nb = _lib.PARAM_BUCKETS
+    before = autotune.current()
+    try:
+        for b in range(nb):
+            assert L.diso_set_bucket_param(_lib.PARAM_RUN, b, 1 + b) == 0
+            assert L.diso_set_bucket_param(_lib.PARAM_COUNT_BLOCKS_PER_CU, b, 1 + b % 4) == 0
+        assert [L.diso_get_bucket_param(_lib.PARAM_RUN, b) for b in range(nb)] == [1, 2, 3, 4, 5]
+        assert [L.diso_get_bucket_param(_lib.PARAM_COUNT_BLOCKS_PER_CU, b) for b in range(nb)] == [1, 2, 3, 4, 1]
+        assert L.diso_get_param(_lib.PARAM_RUN) == 4
+        assert L.diso_set_bucket_param(_lib.PARAM_RUN, nb, 8) == -1 and b"bucket" in L.diso_last_error()
+        assert L.diso_set_bucket_param(_lib.PARAM_RUN, -1, 8) == -1
+        assert L.diso_set_bucket_param(_lib.PARAM_RUN, 0, 65) == -1 and b"run length" in L.diso_last_error()
+        assert L.diso_set_bucket_param(_lib.PARAM_RECOUNT_SPINS, 0, 8) == -1
+        assert b"no size buckets" in L.diso_last_error()
+        assert L.diso_get_bucket_param(_lib.PARAM_RUN, nb) == -1
+        autotune.apply({"64": {"run": 8}, "512": {"count_blocks_per_cu": 3}})
+        cur = autotune.current()
+        assert cur["64"]["run"] == 8 and cur["512"]["count_blocks_per_cu"] == 3 and cur["32"]["run"] == 1
+    finally:
+        autotune.apply(before)
+
+
+def test_param_bucket_of_grids():
+    """Cubic grids of the representative sizes land in their own bucket (virtual pad), and the
+    limits follow the segment count ((X+2)(Y+2) rows of ceil((Z+2)/64) segments)."""
+    L = _lib.lib()
+    pad = _lib.FLAG_VIRTUAL_PAD
+    assert [L.diso_param_bucket(n, n, n, pad) for n in (32, 64, 128, 256, 512)] == [0, 1, 2, 3, 4]
+    assert [L.diso_param_bucket(n, n, n, pad) for n in (16, 48, 96, 192, 384, 1024)] == [0, 0, 1, 3, 4, 4]
+    assert L.diso_param_bucket(54, 54, 62, 0) == 0 and L.diso_param_bucket(57, 57, 62, 0) == 1  # 2,916 / 3,249
+    assert L.diso_param_bucket(0, 4, 4, 0) == -1 and b"dims" in L.diso_last_error()
 
 
 def test_batch_descriptor_columns_alias_the_structs():
