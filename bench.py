@@ -23,8 +23,9 @@ device when the autotuner saved one, else the library defaults), the size bucket
 grid and the table of all buckets.
 
 Further legs, keys of the same line: `batched_step` (the headline step as one batched call over
-its two extractions, against the plain step); `round_cube_64` (a 64^3 rounded box, one call at a
-time, for context beside the reference's README table); `batched` (64 grids, N in 32..256) and
+its two extractions, against the plain step); `sphere_64` (the reference example's 64^3 sphere,
+DiffMC forward, one call at a time); `round_cube_64` (a 64^3 rounded box, one call at a time, for
+context beside the reference's README table); `batched` (64 grids, N in 32..256) and
 `batched_small` (512 grids, N in 16..64), each as one batched call per pass against a loop of
 per-grid calls; `serving` (continuous batching under Poisson arrivals); `learn` (graph-captured
 512^3 fit).
@@ -474,6 +475,26 @@ def cpu_model():
     return None
 
 
+def _per_call(mod, sdf0, mode, reps):
+    """End-to-end host time per call (synchronised) of `mod` on sdf0, forward alone or
+    forward+backward; p50 and p99 over `reps` calls after 3 warm-up calls, and the output sizes."""
+    import torch
+
+    ts = []
+    for r in range(reps + 3):
+        sdf = sdf0.detach().requires_grad_(mode == "fwd_bwd")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        v, f = mod(sdf, None, 0.0)
+        if mode == "fwd_bwd":
+            v.sum().backward()
+        torch.cuda.synchronize()
+        if r >= 3:
+            ts.append(time.perf_counter() - t0)
+    return (round(float(np.percentile(ts, 50)) * 1e3, 3), round(float(np.percentile(ts, 99)) * 1e3, 3),
+            int(v.shape[0]), int(f.shape[0]))
+
+
 def round_cube(dev, reps, n=64):
     """For context beside the reference's README table (README.md:79-94, RTX 4090, grid size not
     stated): a rounded box (half-extent 0.35, radius 0.1) at n^3, no deform, one DiffMC and one
@@ -486,26 +507,26 @@ def round_cube(dev, reps, n=64):
 
     sdf0 = torch.tensor(workloads.round_box(n).astype(np.float32), device=dev)
     out = {"grid": n, "workload": f"rounded box half-extent 0.35 radius 0.1 at {n}^3, no deform, end-to-end per call"}
-    for name, mod, kw in (("mc", diso_amd.DiffMC(torch.float32), {}),
-                          ("dmc", diso_amd.DiffDMC(torch.float32), {})):
+    for name, mod in (("mc", diso_amd.DiffMC(torch.float32)), ("dmc", diso_amd.DiffDMC(torch.float32))):
         res = {}
         for mode in ("fwd", "fwd_bwd"):
-            ts = []
-            for r in range(reps + 3):
-                sdf = sdf0.detach().requires_grad_(mode == "fwd_bwd")
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                v, f = mod(sdf, None, 0.0, **kw)
-                if mode == "fwd_bwd":
-                    v.sum().backward()
-                torch.cuda.synchronize()
-                if r >= 3:
-                    ts.append(time.perf_counter() - t0)
-            res[mode + "_p50_ms"] = round(float(np.percentile(ts, 50)) * 1e3, 3)
-            res[mode + "_p99_ms"] = round(float(np.percentile(ts, 99)) * 1e3, 3)
-        res["verts"], res["faces"] = int(v.shape[0]), int(f.shape[0])
+            res[mode + "_p50_ms"], res[mode + "_p99_ms"], res["verts"], res["faces"] = _per_call(mod, sdf0, mode, reps)
         out[name] = res
     return out
+
+
+def sphere_64(dev, reps):
+    """The reference example's input (diso/__init__.py:48-61 usage, test/example.py): |p| - 0.4 on
+    a 64^3 grid over [-0.5, 0.5]^3, fp32, DiffMC forward (normalize=True), end to end per call."""
+    import torch
+
+    import diso_amd
+    from diso_amd import workloads
+
+    sdf0 = torch.tensor(workloads.sphere(64).astype(np.float32), device=dev)
+    p50, p99, nv, nf = _per_call(diso_amd.DiffMC(torch.float32), sdf0, "fwd", reps)
+    return {"workload": "sphere |p| - 0.4 at 64^3, DiffMC forward, end to end per call", "fwd_p50_ms": p50,
+            "fwd_p99_ms": p99, "verts": nv, "faces": nf}
 
 
 def main():
@@ -610,6 +631,7 @@ def main():
     }
     out["batched_step"] = batched_step(sets, ups, args.steps, args.warmup)
     if args.cube_reps > 0:
+        out["sphere_64"] = sphere_64(dev, args.cube_reps)
         out["round_cube_64"] = round_cube(dev, args.cube_reps)
     if args.batch_count > 0:
         del sets, mods
