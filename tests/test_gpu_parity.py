@@ -248,9 +248,13 @@ def test_results_do_not_depend_on_launch_parameters():
         # recount spins 0: every count block counts the ranges of the predecessors that have not
         # published when it first looks (the path that keeps the scan live when launches share
         # the device)
-        for run, cpc, rs in ((1, 1, spins), (7, 3, 0), (32, 4, spins), (64, 2, 0), (16, 4, 0)):
+        for run, cpc, rs in ((1, 1, spins), (7, 3, 0), (32, 4, spins), (64, 2, 0), (16, 4, 0), (3, 1, spins)):
             print(f"case launch run={run} count_blocks_per_cu={cpc} recount_spins={rs}", flush=True)
-            autotune.apply({"run": run, "count_blocks_per_cu": cpc})
+            if run == 3:  # only the bucket these grids and the batch fall in (64^3-class) differs
+                autotune.apply({"run": 48, "count_blocks_per_cu": 4})
+                autotune.apply({"64": {"run": run, "count_blocks_per_cu": cpc}})
+            else:
+                autotune.apply({"run": run, "count_blocks_per_cu": cpc})
             assert L.diso_set_param(diso_amd._lib.PARAM_RECOUNT_SPINS, rs) == 0
             res = []
             for mod in (diso_amd.DiffMC(), diso_amd.DiffDMC()):
