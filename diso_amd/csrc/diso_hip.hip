@@ -477,6 +477,7 @@ struct BatchGrid {
     const S *adj;  // upstream vertex gradients [nv, 3]
     S *adj_sdf, *adj_deform;
     int64_t q_first;  // DMC: first row of its per-edge gradients in the batch scratch
+    int run;          // segments per emission / backward block: the run length of the grid's size bucket
 };
 
 // j such that rf[j] <= b < rf[j + 1] (rf ascending, rf[0] = 0, rf[m] = total).
@@ -1832,7 +1833,7 @@ __global__ __launch_bounds__(kBlock, 7) void emit_mc_batch_kernel(const BatchGri
                                                                const int32_t *__restrict__ gl,
                                                                const int64_t *__restrict__ rf, int m,
                                                                const SegBase *__restrict__ seg,
-                                                               const uint16_t *__restrict__ info, int run, int64_t nblocks) {
+                                                               const uint16_t *__restrict__ info, int64_t nblocks) {
     __shared__ uint64_t s_case[256];
     __shared__ McEmitLds s_lds[kWavesPerBlock];
     __shared__ uint32_t s_take;
@@ -1847,7 +1848,7 @@ __global__ __launch_bounds__(kBlock, 7) void emit_mc_batch_kernel(const BatchGri
     const In<S> in = G.in;
     const SegCtx sc = batch_ctx(G, seg, info);
     const OutCfg out{G.flags, G.vcap, G.fcap};
-    const int64_t lo = (b - rf[j]) * run, hi = lo + run < in.geo.nseg ? lo + run : in.geo.nseg;
+    const int64_t lo = (b - rf[j]) * G.run, hi = lo + G.run < in.geo.nseg ? lo + G.run : in.geo.nseg;
     const uint64_t act = run_active(sc.seg, lo, hi, lane);
     const SegPos lp = seg_pos(in.geo, lo + lane);
     run_segments(
@@ -1862,7 +1863,7 @@ __global__ __launch_bounds__(kBlock, 6) void emit_dmc_batch_kernel(const BatchGr
                                                                 const int32_t *__restrict__ gl,
                                                                 const int64_t *__restrict__ rf, int m,
                                                                 const SegBase *__restrict__ seg,
-                                                                const uint16_t *__restrict__ info, int run, int64_t nblocks) {
+                                                                const uint16_t *__restrict__ info, int64_t nblocks) {
     __shared__ DmcEmitLds<S> s_lds[kWavesPerBlock];
     __shared__ uint32_t s_take;
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
@@ -1875,7 +1876,7 @@ __global__ __launch_bounds__(kBlock, 6) void emit_dmc_batch_kernel(const BatchGr
     const In<S> in = G.in;
     const SegCtx sc = batch_ctx(G, seg, info);
     const OutCfg out{G.flags, G.vcap, G.fcap};
-    const int64_t lo = (b - rf[j]) * run, hi = lo + run < in.geo.nseg ? lo + run : in.geo.nseg;
+    const int64_t lo = (b - rf[j]) * G.run, hi = lo + G.run < in.geo.nseg ? lo + G.run : in.geo.nseg;
     const uint64_t act = run_active(sc.seg, lo, hi, lane);
     const SegPos lp = seg_pos(in.geo, lo + lane);
     run_segments(
@@ -1909,7 +1910,7 @@ __global__ __launch_bounds__(kBlock) void bwd_batch_kernel(const BatchGrid<S> *_
                                                            const int64_t *__restrict__ rf, int m,
                                                            const SegBase *__restrict__ seg,
                                                            const uint16_t *__restrict__ info,
-                                                           const S *__restrict__ ge, int run, int64_t nblocks) {
+                                                           const S *__restrict__ ge, int64_t nblocks) {
     __shared__ BwdLds<S> s_lds[kWavesPerBlock];
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     const int64_t b = xcd_block(blockIdx.x, nblocks);
@@ -1921,7 +1922,7 @@ __global__ __launch_bounds__(kBlock) void bwd_batch_kernel(const BatchGrid<S> *_
     // DMC grids gather the per-edge gradients (already divided by the normalisation)
     const AdjIn<S> adj = G.algo == DISO_DMC ? AdjIn<S>{ge + 3 * G.q_first, G.nf, G.flags & ~DISO_FLAG_NORMALIZE, G.nm}
                                             : AdjIn<S>{G.adj, G.nv, G.flags, G.nm};
-    const int64_t lo = (b - rf[j]) * run, hi = lo + run < g.nseg ? lo + run : g.nseg;
+    const int64_t lo = (b - rf[j]) * G.run, hi = lo + G.run < g.nseg ? lo + G.run : g.nseg;
     const int64_t t = lo + lane;
     const SegPos lp = seg_pos(g, t);
     const bool touched = t < hi && seg_touched(sc.seg, g, t, lp);
@@ -2475,6 +2476,7 @@ std::vector<BatchGrid<S>> batch_desc(const diso_grid_t *grids, const BatchLayout
         D.adj_sdf = (S *)G.adj_sdf;
         D.adj_deform = (S *)G.adj_deform;
         D.q_first = q;
+        D.run = run_for(L.geo[i].nseg);
         if (G.algo == DISO_DMC) q += G.totals.n_faces;
     }
     return d;
@@ -2611,7 +2613,6 @@ int64_t runs_of(const Geo &g, int run) { return (g.nseg + run - 1) / run; }
 
 template <typename S>
 int batch_emit(const diso_grid_t *grids, const BatchLayout &L, void *ws, hipStream_t st) {
-    const int run = run_for(L.nseg);
     Group gr[4];  // MC, MC + deform, DMC, DMC + deform
     for (int i = 0; i < L.n; ++i) {
         const diso_grid_t &G = grids[i];
@@ -2620,7 +2621,7 @@ int batch_emit(const diso_grid_t *grids, const BatchLayout &L, void *ws, hipStre
             return fail(DISO_E_CAPACITY, "grid %d: output buffers too small", i);
         if (!G.verts || !G.faces || (G.algo == DISO_DMC && !G.vinfo))
             return fail(DISO_E_INVALID, "grid %d: null output buffer", i);
-        gr[2 * (G.algo == DISO_DMC) + (G.deform != nullptr)].add(i, runs_of(L.geo[i], run));
+        gr[2 * (G.algo == DISO_DMC) + (G.deform != nullptr)].add(i, runs_of(L.geo[i], run_for(L.geo[i].nseg)));
     }
     int rc = upload_batch<S>(ws, L, batch_desc<S>(grids, L), gr, 4, st);
     if (rc) return rc;
@@ -2634,10 +2635,10 @@ int batch_emit(const diso_grid_t *grids, const BatchLayout &L, void *ws, hipStre
         const int64_t *rf = P.rf(L, k, w);
         const int m = gr[k].m();
         switch (k) {
-            case 0: hipLaunchKernelGGL((emit_mc_batch_kernel<S, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, run, nb); break;
-            case 1: hipLaunchKernelGGL((emit_mc_batch_kernel<S, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, run, nb); break;
-            case 2: hipLaunchKernelGGL((emit_dmc_batch_kernel<S, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, run, nb); break;
-            default: hipLaunchKernelGGL((emit_dmc_batch_kernel<S, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, run, nb); break;
+            case 0: hipLaunchKernelGGL((emit_mc_batch_kernel<S, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, nb); break;
+            case 1: hipLaunchKernelGGL((emit_mc_batch_kernel<S, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, nb); break;
+            case 2: hipLaunchKernelGGL((emit_dmc_batch_kernel<S, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, nb); break;
+            default: hipLaunchKernelGGL((emit_dmc_batch_kernel<S, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, nb); break;
         }
         rc = check_hip(hipGetLastError(), "batch emit kernel launch");
         if (rc) return rc;
@@ -2654,7 +2655,6 @@ int64_t batch_scratch_bytes(int dtype, const diso_grid_t *grids, int n) {
 
 template <typename S>
 int batch_backward(const diso_grid_t *grids, const BatchLayout &L, void *ws, void *scratch, hipStream_t st) {
-    const int run = run_for(L.nseg);
     Group gr[5];  // 0-3: gather, deform x recompute; 4: DMC per-edge gradients
     for (int i = 0; i < L.n; ++i) {
         const diso_grid_t &G = grids[i];
@@ -2665,7 +2665,7 @@ int batch_backward(const diso_grid_t *grids, const BatchLayout &L, void *ws, voi
             if (!G.faces || !G.vinfo) return fail(DISO_E_INVALID, "grid %d: DMC backward needs quads and vinfo", i);
             gr[4].add(i, (G.totals.n_faces + kBlock - 1) / kBlock);
         }
-        gr[2 * (G.deform != nullptr) + ((G.flags & DISO_FLAG_RECOMPUTE) != 0)].add(i, runs_of(L.geo[i], run));
+        gr[2 * (G.deform != nullptr) + ((G.flags & DISO_FLAG_RECOMPUTE) != 0)].add(i, runs_of(L.geo[i], run_for(L.geo[i].nseg)));
     }
     int rc = upload_batch<S>(ws, L, batch_desc<S>(grids, L), gr, 5, st);
     if (rc) return rc;
@@ -2686,10 +2686,10 @@ int batch_backward(const diso_grid_t *grids, const BatchLayout &L, void *ws, voi
         const int64_t *rf = P.rf(L, k, w);
         const int m = gr[k].m();
         switch (k) {
-            case 0: hipLaunchKernelGGL((bwd_batch_kernel<S, false, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, run, nb); break;
-            case 1: hipLaunchKernelGGL((bwd_batch_kernel<S, false, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, run, nb); break;
-            case 2: hipLaunchKernelGGL((bwd_batch_kernel<S, true, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, run, nb); break;
-            default: hipLaunchKernelGGL((bwd_batch_kernel<S, true, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, run, nb); break;
+            case 0: hipLaunchKernelGGL((bwd_batch_kernel<S, false, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, nb); break;
+            case 1: hipLaunchKernelGGL((bwd_batch_kernel<S, false, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, nb); break;
+            case 2: hipLaunchKernelGGL((bwd_batch_kernel<S, true, false>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, nb); break;
+            default: hipLaunchKernelGGL((bwd_batch_kernel<S, true, true>), grid, block, 0, st, P.desc, gl, rf, m, P.seg, P.info, ge, nb); break;
         }
         rc = check_hip(hipGetLastError(), "batch backward kernel launch");
         if (rc) return rc;
