@@ -3,8 +3,9 @@
 The engine has two launch parameters per size bucket (include/diso_hip.h, diso_set_bucket_param):
 the number of segments each emission / backward block takes (the "run", 1..64) and the number of
 resident count blocks per CU (1..4). A launch takes the bucket of its segment count; buckets hold
-grids of about 32^3, 64^3, 128^3, 256^3 and 512^3 samples (a batched launch: all its grids
-together). Results never depend on the parameters; speed does, by device and grid size.
+grids of about 32^3, 64^3, 128^3, 256^3 and 512^3 samples (a batched count: all its grids
+together; batched emission and backward: each grid's own run length). Results never depend on
+the parameters; speed does, by device and grid size.
 
 `tune()` times, per bucket, the count launches of DiffMC + DiffDMC for every count-blocks choice
 and their emission + backward launches for every run length (the two parameters act on different

@@ -218,7 +218,8 @@ int diso_batch_backward(int dtype, const diso_grid_t *grids, int n, void *seg, i
  *    counts that range itself, 0..2^20 (default 8192; 0 counts every predecessor not already
  *    published, which tests use to exercise that path).
  * The first two have one value per size bucket. A launch takes the bucket of its segment count
- * (a batched launch: of all its grids together); buckets 0..4 hold grids of about 32^3, 64^3,
+ * (a batched count: of all its grids together; a batched emission or backward: each grid its own
+ * run length); buckets 0..4 hold grids of about 32^3, 64^3,
  * 128^3, 256^3 and 512^3 samples (limits 3,200 / 21,000 / 130,000 / 900,000 segments).
  * diso_set_param sets every bucket; diso_get_param returns bucket 3's value. */
 enum {
