@@ -208,6 +208,47 @@ __device__ __forceinline__ int dmc_good_code(const In<S> &in, const uint64_t *ca
     return code;
 }
 
+// dmc_good_code for a cell whose eight corners are all stored samples (`own` = index of corner
+// (0,0,0)): the face neighbour shares four corners with the cell, whose bits come from `code`
+// (Morton order: bit (dx | dy << 1 | dz << 2)); only its far face, the plane two steps past the
+// cell (step +1) or one step before it (step -1), is read, at the cell's index plus strides.
+template <typename S>
+__device__ __forceinline__ int dmc_good_code_stored(const In<S> &in, const uint64_t *cases, int code, int x, int y,
+                                                    int z, int64_t own) {
+    const int dir = DISO_DMC_DIR(cases[code]);
+    if (dir == 7) return code;
+    const Geo &g = in.geo;
+    const int comp = dir >> 1, plus = dir & 1;
+    const int c = comp == 0 ? x : (comp == 1 ? y : z);
+    const int lim = comp == 0 ? g.g0 : (comp == 1 ? g.g1 : g.g2);
+    const int nc = plus ? c + 1 : c - 1;  // the neighbour's anchor along comp
+    if (nc < 0 || nc >= lim - 1) return code;
+    const int64_t sz = 1, sy = g.n2, sx = (int64_t)g.n1 * g.n2;
+    // strides of comp and of the face's two other axes a < b
+    const int64_t sc = comp == 0 ? sx : (comp == 1 ? sy : sz);
+    const int64_t sa = comp == 0 ? sy : sx, sb = comp == 2 ? sy : sz;
+    const int a = comp == 0 ? 1 : 0, b = comp == 2 ? 1 : 2;
+    // far plane: conceptual c + 2 (step +1) or c - 1 (step -1); outside the stored grid it is pad
+    const int fc = plus ? c + 2 : c - 1;
+    const int nfc = comp == 0 ? g.n0 : (comp == 1 ? g.n1 : g.n2);
+    const bool stored = (unsigned)(fc - g.off) < (unsigned)nfc;
+    const int64_t f0 = own + (plus ? 2 * sc : -sc);
+    const int wc = 1 << comp;  // bit weight of comp in the corner index
+    constexpr int kLow[3] = {0x55, 0x33, 0x0f};  // corners with offset 0 along x, y, z
+    const int low = comp == 0 ? kLow[0] : (comp == 1 ? kLow[1] : kLow[2]);
+    // shared face: the neighbour's offset-0 corners are the cell's offset-1 ones (step +1), or its
+    // offset-1 corners the cell's offset-0 ones (step -1)
+    int ncode = plus ? ((code >> wc) & low) : ((code << wc) & ~low & 0xff);
+    const int fbit = plus ? wc : 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int ja = j & 1, jb = j >> 1;
+        const S v = stored ? in.sdf[f0 + ja * sa + jb * sb] : in.pad;
+        ncode |= (int)(v >= in.iso) << (fbit | (ja << a) | (jb << b));
+    }
+    return DISO_DMC_DIR(cases[ncode]) != 7 ? code ^ 0xff : code;
+}
+
 __device__ __forceinline__ int popc3(int m) { return (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1); }
 
 // ------------------------------------------------------------------------------------------
@@ -563,7 +604,9 @@ __device__ __forceinline__ void count_position(const In<S> &in, int ALGO, const 
         code = 0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) code |= (w(i & 1, (i >> 1) & 1, (i >> 2) & 1) >= in.iso) << i;
-        if (code != 0 && code != 255) code = dmc_good_code(in, cases, code, px, py, pz);
+        if (code != 0 && code != 255)
+            code = w.interior ? dmc_good_code_stored(in, cases, code, px, py, pz, data_index(in, px, py, pz))
+                              : dmc_good_code(in, cases, code, px, py, pz);
         *nf = DISO_DMC_NPATCH(cases[code]);
     }
     *info |= (uint32_t)code | ((uint32_t)m << 8) | ((uint32_t)*nf << 12);
