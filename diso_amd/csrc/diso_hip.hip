@@ -644,11 +644,98 @@ struct CountSrc {
 // adds the wave's totals to (wv, wf, wfl). WRITE: also record each segment's counts (lane 0),
 // every lane's classification word and (BATCH) the grids' sample flags; without WRITE the range
 // is only summed (a block recounting a predecessor's range, see the look-back).
+// count_range for a single grid whose last chunk of each row holds at most 32 positions (the
+// z extent is a few positions past a multiple of 64, as 256 + 2 pad samples is): the range's full
+// segments one per wave iteration, then its last-chunk segments ("tails") several per iteration,
+// P lanes each (P = tail positions rounded up to a power of two), so a wave iteration is not spent
+// on a few positions. A tail's remaining words (positions past the grid) all hold the pad sample's
+// inside bit. Same counts, words and flags as one segment per iteration.
+template <typename S, int ALGO, bool WRITE>
+__device__ __forceinline__ void count_range_tails(const In<S> &in, SegBase *seg, uint16_t *info, const uint64_t *cases,
+                                                  int64_t wlo, int64_t whi, int lane, int64_t &wv, int64_t &wf,
+                                                  uint32_t &wfl) {
+    const Geo &g = in.geo;
+    const int nc = g.nchunk, last = nc - 1, tailr = g.g2 - kWave * last;
+    uint32_t gfl = 0;
+    SegPos sp = seg_pos(g, wlo);
+    for (int64_t s = wlo; s < whi; ++s) {  // wave-uniform
+        if (sp.chunk != last) {
+            int nv, nf;
+            uint32_t fl, iw;
+            CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
+            w.template load_box<0, 2, 0, 2, 0, 2>(in);
+            count_position<S>(in, ALGO, cases, w, &nv, &nf, &fl, &iw);
+            const int tv = wave_total_small(nv), tf = wave_total_small(nf);
+            if constexpr (WRITE) {
+                info[s * kWave + lane] = (uint16_t)iw;
+                if (lane == 0) {
+                    seg[s].v = tv;
+                    seg[s].f = tf;
+                }
+            }
+            wv += tv;
+            wf += tf;
+            gfl |= fl;
+        }
+        if (++sp.chunk == nc) {
+            sp.chunk = 0;
+            if (++sp.py == g.g1) {
+                sp.py = 0;
+                ++sp.px;
+            }
+        }
+    }
+    // tails of the range: segments t0, t0 + nc, ... below whi
+    const int64_t t0 = wlo + ((last - wlo % nc) + nc) % nc;
+    const int64_t nt = t0 < whi ? (whi - 1 - t0) / nc + 1 : 0;
+    int P = 1;
+    while (P < tailr) P <<= 1;
+    const int per = kWave / P, t = lane / P, i = lane % P;
+    const uint16_t beyond = (uint16_t)((uint32_t)(in.pad >= in.iso) << 11);  // word of a position past the grid
+    for (int64_t b = 0; b < nt; b += per) {  // wave-uniform
+        const bool valid = b + t < nt && i < tailr;
+        const int64_t st = t0 + (b + t < nt ? b + t : 0) * nc;
+        const SegPos tp = seg_pos(g, st);  // per lane
+        int nv, nf;
+        uint32_t fl, iw;
+        CornerWin<S> w(in, tp.px, tp.py, last * kWave + i, last * kWave);
+        w.template load_box<0, 2, 0, 2, 0, 2>(in);
+        count_position<S>(in, ALGO, cases, w, &nv, &nf, &fl, &iw);
+        if (!valid) nv = nf = 0, fl = 0;
+        int sv = nv, sf = nf;  // per-tail sums over its P lanes
+        for (int o = 1; o < P; o <<= 1) {
+            sv += __shfl_xor(sv, o, kWave);
+            sf += __shfl_xor(sf, o, kWave);
+        }
+        if constexpr (WRITE) {
+            if (valid) info[st * kWave + i] = (uint16_t)iw;
+            if (valid && i == 0) {
+                seg[st].v = sv;
+                seg[st].f = sf;
+            }
+            const int ng = nt - b < per ? (int)(nt - b) : per, cw = kWave - tailr;
+            for (int k = lane; k < ng * cw; k += kWave)
+                info[(t0 + (b + k / cw) * nc) * kWave + tailr + k % cw] = beyond;
+        }
+        wv += wave_total_small(nv);
+        wf += wave_total_small(nf);
+        gfl |= fl;
+    }
+    wfl |= gfl;
+}
+
 template <typename S, int ALGO, bool BATCH, bool WRITE>
 __device__ __forceinline__ void count_range(const CountSrc<S> &src, const BatchGrid<S> *__restrict__ grids,
                                             SegBase *seg, uint16_t *info, const uint64_t (*s_case)[256],
                                             int64_t wlo, int64_t whi, int lane, int64_t &wv, int64_t &wf,
                                             uint32_t &wfl) {
+    if constexpr (!BATCH) {
+        const Geo &g = src.in.geo;
+        if (g.nchunk >= 2 && g.g2 - kWave * (g.nchunk - 1) <= kWave / 2) {
+            count_range_tails<S, ALGO, WRITE>(src.in, seg, info, s_case[0], wlo, whi, lane, wv, wf, wfl);
+            return;
+        }
+    }
     // BATCH: the range may span several grids; one pass of the outer loop per grid, with that
     // grid's parameters held in scalar registers for the inner loop.
     int gi = 0;
