@@ -644,27 +644,34 @@ struct CountSrc {
 // adds the wave's totals to (wv, wf, wfl). WRITE: also record each segment's counts (lane 0),
 // every lane's classification word and (BATCH) the grids' sample flags; without WRITE the range
 // is only summed (a block recounting a predecessor's range, see the look-back).
-// count_range for a single grid whose last chunk of each row holds at most 32 positions (the
-// z extent is a few positions past a multiple of 64, as 256 + 2 pad samples is): the range's full
-// segments one per wave iteration, then its last-chunk segments ("tails") several per iteration,
-// P lanes each (P = tail positions rounded up to a power of two), so a wave iteration is not spent
-// on a few positions. A tail's remaining words (positions past the grid) all hold the pad sample's
-// inside bit. Same counts, words and flags as one segment per iteration.
-template <typename S, int ALGO, bool WRITE>
-__device__ __forceinline__ void count_range_tails(const In<S> &in, SegBase *seg, uint16_t *info, const uint64_t *cases,
-                                                  int64_t wlo, int64_t whi, int lane, int64_t &wv, int64_t &wf,
-                                                  uint32_t &wfl) {
+// Whether count_range_tails applies to a grid: rows of at least two chunks whose last chunk holds
+// at most 32 positions (the z extent a few positions past a multiple of 64, as 256 + 2 pad
+// samples is).
+__device__ __forceinline__ bool short_tails(const Geo &g) {
+    return g.nchunk >= 2 && g.g2 - kWave * (g.nchunk - 1) <= kWave / 2;
+}
+
+// Counts the segments [wlo, whi) of one grid whose segment 0 is segment `first` of the launch
+// (a short_tails grid): its full segments one per wave iteration, then its last-chunk segments
+// ("tails") several per iteration, P lanes each (P = tail positions rounded up to a power of two),
+// so that a wave iteration is not spent on a few positions. A tail's remaining words (positions
+// past the grid) all hold the pad sample's inside bit. Same counts, words and flags as one segment
+// per iteration; returns the grid's sample flags of the range.
+template <typename S, bool WRITE>
+__device__ __forceinline__ uint32_t count_range_tails(const In<S> &in, int algo, int64_t first, SegBase *seg,
+                                                      uint16_t *info, const uint64_t *cases, int64_t wlo, int64_t whi,
+                                                      int lane, int64_t &wv, int64_t &wf) {
     const Geo &g = in.geo;
     const int nc = g.nchunk, last = nc - 1, tailr = g.g2 - kWave * last;
     uint32_t gfl = 0;
-    SegPos sp = seg_pos(g, wlo);
+    SegPos sp = seg_pos(g, wlo - first);
     for (int64_t s = wlo; s < whi; ++s) {  // wave-uniform
         if (sp.chunk != last) {
             int nv, nf;
             uint32_t fl, iw;
             CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
             w.template load_box<0, 2, 0, 2, 0, 2>(in);
-            count_position<S>(in, ALGO, cases, w, &nv, &nf, &fl, &iw);
+            count_position<S>(in, algo, cases, w, &nv, &nf, &fl, &iw);
             const int tv = wave_total_small(nv), tf = wave_total_small(nf);
             if constexpr (WRITE) {
                 info[s * kWave + lane] = (uint16_t)iw;
@@ -685,8 +692,8 @@ __device__ __forceinline__ void count_range_tails(const In<S> &in, SegBase *seg,
             }
         }
     }
-    // tails of the range: segments t0, t0 + nc, ... below whi
-    const int64_t t0 = wlo + ((last - wlo % nc) + nc) % nc;
+    // tails of the range: segments t0, t0 + nc, ... below whi (t0 - first = last mod nc)
+    const int64_t t0 = wlo + ((last - (wlo - first) % nc) + nc) % nc;
     const int64_t nt = t0 < whi ? (whi - 1 - t0) / nc + 1 : 0;
     int P = 1;
     while (P < tailr) P <<= 1;
@@ -695,12 +702,12 @@ __device__ __forceinline__ void count_range_tails(const In<S> &in, SegBase *seg,
     for (int64_t b = 0; b < nt; b += per) {  // wave-uniform
         const bool valid = b + t < nt && i < tailr;
         const int64_t st = t0 + (b + t < nt ? b + t : 0) * nc;
-        const SegPos tp = seg_pos(g, st);  // per lane
+        const SegPos tp = seg_pos(g, st - first);  // per lane
         int nv, nf;
         uint32_t fl, iw;
         CornerWin<S> w(in, tp.px, tp.py, last * kWave + i, last * kWave);
         w.template load_box<0, 2, 0, 2, 0, 2>(in);
-        count_position<S>(in, ALGO, cases, w, &nv, &nf, &fl, &iw);
+        count_position<S>(in, algo, cases, w, &nv, &nf, &fl, &iw);
         if (!valid) nv = nf = 0, fl = 0;
         int sv = nv, sf = nf;  // per-tail sums over its P lanes
         for (int o = 1; o < P; o <<= 1) {
@@ -721,7 +728,7 @@ __device__ __forceinline__ void count_range_tails(const In<S> &in, SegBase *seg,
         wf += wave_total_small(nf);
         gfl |= fl;
     }
-    wfl |= gfl;
+    return gfl;
 }
 
 template <typename S, int ALGO, bool BATCH, bool WRITE>
@@ -730,9 +737,8 @@ __device__ __forceinline__ void count_range(const CountSrc<S> &src, const BatchG
                                             int64_t wlo, int64_t whi, int lane, int64_t &wv, int64_t &wf,
                                             uint32_t &wfl) {
     if constexpr (!BATCH) {
-        const Geo &g = src.in.geo;
-        if (g.nchunk >= 2 && g.g2 - kWave * (g.nchunk - 1) <= kWave / 2) {
-            count_range_tails<S, ALGO, WRITE>(src.in, seg, info, s_case[0], wlo, whi, lane, wv, wf, wfl);
+        if (short_tails(src.in.geo)) {
+            wfl |= count_range_tails<S, WRITE>(src.in, ALGO, 0, seg, info, s_case[0], wlo, whi, lane, wv, wf);
             return;
         }
     }
@@ -753,6 +759,10 @@ __device__ __forceinline__ void count_range(const CountSrc<S> &src, const BatchG
                                                                                           : whi;
         const uint64_t *cases = s_case[(BATCH && algo != DISO_MC) ? 1 : 0];
         uint32_t gfl = 0;  // the grid's sample flags
+        if (BATCH && short_tails(in.geo)) {
+            gfl = count_range_tails<S, WRITE>(in, algo, first, seg, info, cases, s, end, lane, wv, wf);
+            s = end;
+        }
         SegPos sp = seg_pos(in.geo, s - first);
         for (; s < end; ++s) {
             int nv, nf;
