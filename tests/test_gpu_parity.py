@@ -382,3 +382,38 @@ def test_reused_module_outputs_allow_inplace_edits(algo, quads):
         grads.append((v.detach().clone(), sdf.grad, deform.grad))
     for v, gs, gd in grads[1:]:
         assert torch.equal(v, grads[0][0]) and torch.equal(gs, grads[0][1]) and torch.equal(gd, grads[0][2])
+
+
+# z extents whose padded rows end in a short last chunk (1, 17 and 32 positions: counted several
+# rows per wave iteration) and one just past it (33 positions: one row per iteration)
+TAIL_DIMS = [(6, 5, 63), (5, 7, 79), (7, 4, 94), (4, 6, 95)]
+
+
+@pytest.mark.parametrize("dims", TAIL_DIMS)
+@pytest.mark.parametrize("algo", [O.MC, O.DMC])
+def test_short_row_tails_match_oracle(algo, dims):
+    ref, rv, rf, sdf, deform, verts, faces = _run_both(algo, np.float32, dims, "noise", True)
+    assert np.array_equal(faces.cpu().numpy(), rf)
+    assert np.array_equal(verts.detach().cpu().numpy(), rv)
+    g = np.random.default_rng(5).standard_normal(rv.shape).astype(np.float32)
+    (verts * torch.tensor(g, device=DEV)).sum().backward()
+    gs, gd = ref.backward(g)
+    grad_close(sdf.grad.cpu().numpy(), gs, TOL[np.float32])
+    grad_close(deform.grad.cpu().numpy(), gd, TOL[np.float32])
+
+
+def test_short_row_tails_in_one_batch():
+    """The same grids in one batched call (their tails counted inside the batch's ranges, with
+    the other grids' segments around them): every grid as the oracle has it."""
+    from tests.helpers import check_against_oracle
+
+    data = [inputs(d, "noise", 20 + i, np.float32, True) for i, d in enumerate(TAIL_DIMS)]
+    algos = ["mc", "dmc", "mc", "dmc"]
+    print(f"case batch dims={TAIL_DIMS}", flush=True)
+    sdfs = [torch.tensor(s, device=DEV, requires_grad=True) for s, _ in data]
+    defs = [torch.tensor(d, device=DEV, requires_grad=True) for _, d in data]
+    outs = diso_amd.BatchExtractor()(sdfs, algos, defs, return_quads=True)
+    ups = [torch.randn(v.shape, device=DEV) for v, _ in outs]
+    torch.autograd.backward([v for v, _ in outs], grad_tensors=ups)
+    for i, (v, f) in enumerate(outs):
+        check_against_oracle(algos[i], data[i][0], data[i][1], v, f, ups[i], sdfs[i].grad, defs[i].grad)
