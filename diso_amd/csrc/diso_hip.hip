@@ -1152,7 +1152,8 @@ __device__ __forceinline__ void edge_ids(int32_t *c, int32_t r00, int32_t r10, i
 // Cell codes and crossing masks come from the count pass's classification words.
 struct McEmitLds {
     int32_t cand[kWave][12];  // vertex id of each table edge, relative to the segment's first vertex
-    uint16_t corner[kWave * 15];
+    uint64_t cw[kWave];       // case word of each lane's cell
+    uint16_t tri[kWave * 5];  // triangle items: lane << 3 | triangle of the lane's cell
     uint8_t vert[kWave * 3];
 };
 
@@ -1225,7 +1226,8 @@ __device__ __forceinline__ void emit_mc_segment(const In<S> &in, const SegCtx &s
     }
     if (ntri) {
         edge_ids(L.cand[lane], p00, r10, r01, r11, m00, m10, m01, m11, c00n, c10n);
-        for (int i = 0; i < 3 * ntri; ++i) L.corner[3 * pt + i] = (uint16_t)((lane << 4) | DISO_MC_EDGE(cw, i));
+        L.cw[lane] = cw;
+        for (int t = 0; t < ntri; ++t) L.tri[pt + t] = (uint16_t)((lane << 3) | t);
     }
     wave_lds_sync();
 
@@ -1234,10 +1236,14 @@ __device__ __forceinline__ void emit_mc_segment(const In<S> &in, const SegCtx &s
         const V3<S> p = edge_vertex<S, DEF>(in, px, py, z0 + src, a);
         store_vertex(verts, vbase + k, out.vcap, epilogue(p, out.flags, nm));
     }
+    // corner k of the segment's triangles: corner c of triangle k / 3, whose table edge comes from
+    // its cell's case word
     const int64_t f0 = 3 * tbase, flim = 3 * out.fcap;
     for (int k = lane; k < 3 * tt; k += kWave) {
-        const int it = L.corner[k];
-        if (f0 + k < flim) store_index(faces, f0 + k, vbase + L.cand[it >> 4][it & 15], out.flags);
+        const int tr = k / 3, c = k - 3 * tr;
+        const int it = L.tri[tr], src = it >> 3;
+        const int e = DISO_MC_EDGE(L.cw[src], 3 * (it & 7) + c);
+        if (f0 + k < flim) store_index(faces, f0 + k, vbase + L.cand[src][e], out.flags);
     }
 }
 
