@@ -1,7 +1,7 @@
 """The benchmark configurations' own inputs on the HIP path (needs a GPU), tied to the oracle.
 
 - N=64 sphere ||p|| - 0.4, MC forward, no deform, normalised: bit-exact against the oracle, with
-  the survey's counts (V = 11,952, F = 23,900; DMC Vd = 11,954, Q = 11,952).
+  the expected counts (V = 11,952, F = 23,900; DMC Vd = 11,954, Q = 11,952).
 - N=32 fp64 rounded box + noise with deform: central finite differences (eps 1e-6) of
   L = sum(g * verts) through DiffMC / DiffDMC on the GPU against the GPU backward, on samples whose
   +-eps step changes no topology and whose incident crossing edges are not nearly flat.
