@@ -270,6 +270,22 @@ __device__ __forceinline__ int wave_excl_small(int v, int *total) {
     return lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2);
 }
 
+// Inclusive prefix over the wave of a 32-bit value: a scan within each row of 16 lanes (DPP row
+// shifts by 1, 2, 4, 8; lanes before the row start read 0), then the last lane of row 0 added to
+// rows 1 and 3 and the last lane of row 1 (by then the sum of rows 0 and 1) to rows 2 and 3 (DPP
+// row broadcasts). Six adds. Several small counts can be scanned at once, each in a bit field wide
+// enough for its wave total. Must be reached by all 64 lanes.
+__device__ __forceinline__ uint32_t wave_incl_dpp(uint32_t v) {
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 to rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 to rows 2, 3
+    return (uint32_t)x;
+}
+
 __device__ __forceinline__ int64_t wave_sum64(int64_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
@@ -1436,25 +1452,32 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
     // Crossing masks of the rows the cells' edges start from: 10, 01, 11 (this z) and the halo.
     const int m00 = info_mask(w0), m10 = info_mask(d.w10), m01 = info_mask(d.w01), m11 = info_mask(d.w11);
     const int h00 = info_mask(d.h00), h10 = info_mask(d.h10), h01 = info_mask(d.h01);
-    // prefix of the segment's own patches; also the first vertices of row (0, 0)'s cells below
+    // Exclusive prefixes over the lanes, in two packed scans of 10-bit fields (wave totals <= 448):
+    // the patch counts of rows (0,0) (the segment's own; also the first vertices of row (0,0)'s
+    // cells below), (0,-1), (-1,0) and (-1,-1) at this z, and the owned crossing edges (quads).
     const int np = info_count(w0);
-    int tq, tp;
-    const int pp = wave_excl_small(np, &tp);
+    const uint32_t cA = (uint32_t)np | (uint32_t)info_count(d.cur[1]) << 10 | (uint32_t)info_count(d.cur[2]) << 20;
+    const uint32_t cB = (uint32_t)info_count(d.cur[3]) | (uint32_t)popc3(m00) << 10;
+    const uint32_t iA = wave_incl_dpp(cA), iB = wave_incl_dpp(cB);
+    const uint32_t eA = iA - cA, eB = iB - cB;
+    const uint32_t tA = (uint32_t)__builtin_amdgcn_readlane((int)iA, kWave - 1),
+                   tB = (uint32_t)__builtin_amdgcn_readlane((int)iB, kWave - 1);
+    const int pp = (int)(eA & 0x3ff), tp = (int)(tA & 0x3ff);
+    const int pq = (int)((eB >> 10) & 0x3ff), tq = (int)((tB >> 10) & 0x3ff);
+    const int rowpre[4] = {pp, (int)((eA >> 10) & 0x3ff), (int)((eA >> 20) & 0x3ff), (int)(eB & 0x3ff)};
     // Codes and first vertices of the cells (-i, -j, 0) and (-i, -j, -1).
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const uint32_t cur = d.cur[2 * i + j], prev = d.prev[2 * i + j];
-            int t;
-            const int b = (i || j) ? wave_excl_small(info_count(cur), &t) : pp;  // cur of (0, 0) is w0
+            const int b = rowpre[2 * i + j];
             if (lane == 0) L.rowb[2 * i + j] = d.rowd[2 * i + j];
             L.code[4 * i + 2 * j][lane] = (uint8_t)info_code(cur);
             L.pre[4 * i + 2 * j][lane] = (int16_t)b;
             L.code[4 * i + 2 * j + 1][lane] = (uint8_t)info_code(prev);
             L.pre[4 * i + 2 * j + 1][lane] = (int16_t)(b - info_count(prev));
         }
-    const int pq = wave_excl_small(popc3(m00), &tq);
     for (int k = 0; k < np; ++k) L.patch[pp + k] = (uint8_t)(lane * 4 + k);
     {
         // an owned crossing edge leaves the inside when the position's sample is >= iso
