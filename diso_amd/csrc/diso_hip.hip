@@ -1227,12 +1227,19 @@ __device__ __forceinline__ void emit_mc_segment(const In<S> &in, const SegCtx &s
     const uint64_t cw = s_case[info_code(w00)];  // Bourke corner order (src/cumc.cu:107)
     const int ntri = DISO_MC_NTRI(cw);
 
-    int tv, tt, t;
-    const int p00 = wave_excl_small(popc3(m00), &tv);
-    const int pt = wave_excl_small(ntri, &tt);
-    const int32_t r10 = d.d10 + wave_excl_small(popc3(m10), &t);
-    const int32_t r01 = d.d01 + wave_excl_small(popc3(m01), &t);
-    const int32_t r11 = d.d11 + wave_excl_small(popc3(m11), &t);
+    // exclusive prefixes over the lanes of the rows' vertex counts and of the triangle counts, in
+    // two packed scans of 10-bit fields (wave totals <= 320)
+    const uint32_t cA = (uint32_t)popc3(m00) | (uint32_t)popc3(m10) << 10 | (uint32_t)popc3(m01) << 20;
+    const uint32_t cB = (uint32_t)ntri | (uint32_t)popc3(m11) << 10;
+    const uint32_t iA = wave_incl_dpp(cA), iB = wave_incl_dpp(cB);
+    const uint32_t eA = iA - cA, eB = iB - cB;
+    const uint32_t tA = (uint32_t)__builtin_amdgcn_readlane((int)iA, kWave - 1),
+                   tB = (uint32_t)__builtin_amdgcn_readlane((int)iB, kWave - 1);
+    const int p00 = (int)(eA & 0x3ff), tv = (int)(tA & 0x3ff);
+    const int pt = (int)(eB & 0x3ff), tt = (int)(tB & 0x3ff);
+    const int32_t r10 = d.d10 + (int32_t)((eA >> 10) & 0x3ff);
+    const int32_t r01 = d.d01 + (int32_t)((eA >> 20) & 0x3ff);
+    const int32_t r11 = d.d11 + (int32_t)((eB >> 10) & 0x3ff);
 
     {
         int r = p00;
@@ -1819,11 +1826,16 @@ __device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, c
     // slots 0-2: edges the vertex owns; 3-5: the x edge of -x, the y edge of -y, the z edge of -z
     const int inc_m = (mm0 & 1) | (m0m & 2) | (mzm & 4);
     const int mask = in_data(in, px, py, pz) ? (m00 | (inc_m << 3)) : 0;
-    int t, total;
-    const int32_t r00 = wave_excl_small(popc3(m00), &t);
-    const int32_t rm0 = sg.dm0 + wave_excl_small(popc3(mm0), &t);
-    const int32_t r0m = sg.d0m + wave_excl_small(popc3(m0m), &t);
-    const int first = wave_excl_small(__popc(mask), &total);
+    // exclusive prefixes over the lanes of the rows' vertex counts (10-bit fields, totals <= 192)
+    // and of the items, in two DPP scans
+    const uint32_t cA = (uint32_t)popc3(m00) | (uint32_t)popc3(mm0) << 10 | (uint32_t)popc3(m0m) << 20;
+    const uint32_t cB = (uint32_t)__popc(mask);
+    const uint32_t iA = wave_incl_dpp(cA), iB = wave_incl_dpp(cB);
+    const uint32_t eA = iA - cA;
+    const int32_t r00 = (int32_t)(eA & 0x3ff);
+    const int32_t rm0 = sg.dm0 + (int32_t)((eA >> 10) & 0x3ff);
+    const int32_t r0m = sg.d0m + (int32_t)((eA >> 20) & 0x3ff);
+    const int first = (int)(iB - cB), total = __builtin_amdgcn_readlane((int)iB, kWave - 1);
 
     L.id[0][lane] = r00;
     L.id[1][lane] = r00 + (m00 & 1);
