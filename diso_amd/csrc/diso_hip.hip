@@ -262,14 +262,6 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Exclusive prefix over the wave of a per-lane value in [0, 7] (three ballots); *total gets the
-// wave sum. Must be reached by all 64 lanes.
-__device__ __forceinline__ int wave_excl_small(int v, int *total) {
-    uint64_t b0 = __ballot(v & 1), b1 = __ballot(v & 2), b2 = __ballot(v & 4);
-    *total = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
-    return lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2);
-}
-
 // Inclusive prefix over the wave of a 32-bit value: a scan within each row of 16 lanes (DPP row
 // shifts by 1, 2, 4, 8; lanes before the row start read 0), then the last lane of row 0 added to
 // rows 1 and 3 and the last lane of row 1 (by then the sum of rows 0 and 1) to rows 2 and 3 (DPP
@@ -628,9 +620,11 @@ __device__ __forceinline__ void count_position(const In<S> &in, int ALGO, const 
     *info |= (uint32_t)code | ((uint32_t)m << 8) | ((uint32_t)*nf << 12);
 }
 
-// Wave sum of a per-lane value in [0, 7] (three ballots).
-__device__ __forceinline__ int wave_total_small(int v) {
-    return __popcll(__ballot(v & 1)) + 2 * __popcll(__ballot(v & 2)) + 4 * __popcll(__ballot(v & 4));
+// Wave sums of two per-lane counts in [0, 7]: one DPP scan of both as 10-bit fields.
+__device__ __forceinline__ void wave_totals2(int a, int b, int *ta, int *tb) {
+    const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_dpp((uint32_t)a | (uint32_t)b << 10), kWave - 1);
+    *ta = (int)(t & 0x3ff);
+    *tb = (int)((t >> 10) & 0x3ff);
 }
 
 __device__ __forceinline__ int64_t wave_incl64(int64_t v) {
@@ -688,7 +682,8 @@ __device__ __forceinline__ uint32_t count_range_tails(const In<S> &in, int algo,
             CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
             w.template load_box<0, 2, 0, 2, 0, 2>(in);
             count_position<S>(in, algo, cases, w, &nv, &nf, &fl, &iw);
-            const int tv = wave_total_small(nv), tf = wave_total_small(nf);
+            int tv, tf;
+            wave_totals2(nv, nf, &tv, &tf);
             if constexpr (WRITE) {
                 info[s * kWave + lane] = (uint16_t)iw;
                 if (lane == 0) {
@@ -740,8 +735,10 @@ __device__ __forceinline__ uint32_t count_range_tails(const In<S> &in, int algo,
             for (int k = lane; k < ng * cw; k += kWave)
                 info[(t0 + (b + k / cw) * nc) * kWave + tailr + k % cw] = beyond;
         }
-        wv += wave_total_small(nv);
-        wf += wave_total_small(nf);
+        int tv, tf;
+        wave_totals2(nv, nf, &tv, &tf);
+        wv += tv;
+        wf += tf;
         gfl |= fl;
     }
     return gfl;
@@ -793,7 +790,8 @@ __device__ __forceinline__ void count_range(const CountSrc<S> &src, const BatchG
                     ++sp.px;
                 }
             }
-            const int tv = wave_total_small(nv), tf = wave_total_small(nf);
+            int tv, tf;
+            wave_totals2(nv, nf, &tv, &tf);
             if constexpr (WRITE) {
                 info[s * kWave + lane] = (uint16_t)iw;
                 if (lane == 0) {
