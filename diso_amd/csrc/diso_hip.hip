@@ -1497,19 +1497,32 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
     const int kbits = (m00 & 7) | (((m10 >> 1) & 3) << 3) | ((m01 & 1) << 5) | (((m01 >> 2) & 1) << 6) |
                       (((m11 >> 2) & 1) << 7);
     const int hbits = (h00 & 3) | (((h10 >> 1) & 1) << 2) | ((h01 & 1) << 3);
+    // Slots are numbered kind by kind, lanes in order within a kind: the lane's exclusive prefix
+    // per kind from two DPP scans of the kind bits spread into 8-bit fields (x * 0x204081 moves bit
+    // i of a 4-bit x to bit 8i), the kinds' wave totals from the scans' last lane; the halo kinds
+    // (lane 63 only) follow all of them.
+    const uint32_t c0 = ((uint32_t)(kbits & 15) * 0x204081u) & 0x01010101u;
+    const uint32_t c1 = ((uint32_t)(kbits >> 4) * 0x204081u) & 0x01010101u;
+    const uint32_t i0 = wave_incl_dpp(c0), i1 = wave_incl_dpp(c1);
+    const uint32_t e0 = i0 - c0, e1 = i1 - c1;
+    const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)i0, kWave - 1),
+                   t1 = (uint32_t)__builtin_amdgcn_readlane((int)i1, kWave - 1);
+    const int hb = __builtin_amdgcn_readlane(hbits, kWave - 1);
     int nslot = 0, myslot[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-        const bool on = k < 8 ? ((kbits >> k) & 1) : ((hbits >> (k - 8)) & 1);
-        const uint64_t bal = __ballot(on);
-        const int ex = lanes_below(bal);
-        myslot[k] = on ? nslot + ex : -1;
-        if (k < 8)
-            L.slot[k][lane] = (uint8_t)myslot[k];
-        else if (lane == kWave - 1)
-            L.hslot[k - 8] = (uint8_t)myslot[k];
-        nslot += __popcll(bal);
+    for (int k = 0; k < 8; ++k) {
+        const int sh = 8 * (k & 3);
+        const int ex = (int)(((k < 4 ? e0 : e1) >> sh) & 0xff);
+        myslot[k] = ((kbits >> k) & 1) ? nslot + ex : -1;
+        L.slot[k][lane] = (uint8_t)myslot[k];
+        nslot += (int)(((k < 4 ? t0 : t1) >> sh) & 0xff);
     }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        myslot[8 + h] = ((hbits >> h) & 1) ? nslot + __popc(hb & ((1 << h) - 1)) : -1;
+        if (lane == kWave - 1) L.hslot[h] = (uint8_t)myslot[8 + h];
+    }
+    nslot += __popc(hb);
     const bool slots = nslot <= kEdgeSlots;  // wave-uniform
     if (slots) {
 #pragma unroll
