@@ -90,11 +90,11 @@ __device__ __forceinline__ void put(T *p, T v) {
     __builtin_nontemporal_store(v, p);
 }
 
+// (Bitwise &, not &&: no short-circuit branch, so loads guarded by it stay branch-free.)
 template <typename S>
 __device__ __forceinline__ bool in_data(const In<S> &in, int x, int y, int z) {
-    return (unsigned)(x - in.geo.off) < (unsigned)in.geo.n0 &&
-           (unsigned)(y - in.geo.off) < (unsigned)in.geo.n1 &&
-           (unsigned)(z - in.geo.off) < (unsigned)in.geo.n2;
+    return ((unsigned)(x - in.geo.off) < (unsigned)in.geo.n0) & ((unsigned)(y - in.geo.off) < (unsigned)in.geo.n1) &
+           ((unsigned)(z - in.geo.off) < (unsigned)in.geo.n2);
 }
 
 template <typename S>
@@ -104,19 +104,25 @@ __device__ __forceinline__ int64_t data_index(const In<S> &in, int x, int y, int
 
 // sdf sample at conceptual (x,y,z); samples outside the stored grid read as the pad value, so no
 // coordinate can address memory outside the input.
+// (Branch-free: a sample outside the stored grid loads element 0 and is replaced by the pad value,
+// so no lane needs its own execution mask around the load.)
 template <typename S>
 __device__ __forceinline__ S sample(const In<S> &in, int x, int y, int z) {
-    return in_data(in, x, y, z) ? in.sdf[data_index(in, x, y, z)] : in.pad;
+    const bool ok = in_data(in, x, y, z);
+    const S v = in.sdf[ok ? data_index(in, x, y, z) : 0];
+    return ok ? v : in.pad;
 }
 
 template <typename S, bool DEF>
 __device__ __forceinline__ V3<S> deform_at(const In<S> &in, int x, int y, int z) {
     V3<S> r = {S(0), S(0), S(0)};
-    if (DEF && in_data(in, x, y, z)) {
-        const S *d = in.deform + 3 * data_index(in, x, y, z);
-        r.x = d[0];
-        r.y = d[1];
-        r.z = d[2];
+    if (DEF) {
+        const bool ok = in_data(in, x, y, z);
+        const S *d = in.deform + 3 * (ok ? data_index(in, x, y, z) : 0);
+        const S dx = d[0], dy = d[1], dz = d[2];
+        r.x = ok ? dx : S(0);
+        r.y = ok ? dy : S(0);
+        r.z = ok ? dz : S(0);
     }
     return r;
 }
