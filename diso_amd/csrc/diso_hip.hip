@@ -1330,6 +1330,13 @@ __device__ constexpr uint64_t edge_kind_table() {
     for (int e = 0; e < 12; ++e) t |= (uint64_t)(edge_kind(e) | (eoz(e) << 3)) << (5 * e);
     return t;
 }
+// A cell's top edges (dz = 1) are all of kinds 0, 1, 3 or 5.
+__device__ constexpr bool top_kinds_ok() {
+    for (int e = 0; e < 12; ++e)
+        if (eoz(e) && !((1 << edge_kind(e)) & 0x2b)) return false;
+    return true;
+}
+static_assert(top_kinds_ok(), "top edges outside the halo slot entries");
 constexpr int kEdgeSlots = 255;  // MC vertices kept per segment; beyond, patches recompute theirs
 
 template <typename S>
@@ -1339,10 +1346,11 @@ struct DmcEmitLds {
     int32_t rowb[4];           // first DMC vertex of row (-i,-j), relative to the segment's first
     uint8_t patch[kWave * 4];  // patch items: lane * 4 + patch
     uint16_t quad[kWave * 3];  // quad items: lane * 8 + axis * 2 + leaving
-    uint8_t slot[8][kWave];    // slot of the lane's edge of each kind (0xff: does not cross)
-    uint8_t hslot[4];          // slots of the halo kinds (position z0 + 64)
-    uint16_t eitem[kEdgeSlots];  // edge items: lane << 4 | kind
-    S ev[3][kEdgeSlots];       // their MC vertices
+    // slot of the lane's edge of each kind (0xff: does not cross); entry kWave of kinds 0, 1, 3
+    // and 5 holds the halo position z0 + 64's (the only kinds a cell's top edges have)
+    uint8_t slot[8][kWave + 1];
+    uint16_t eitem[kEdgeSlots + 1];  // edge items: lane << 4 | kind; the last takes non-crossing edges' writes
+    S ev[3][kEdgeSlots + 1];         // their MC vertices; the last entry holds zeros
 };
 
 // Edge masks of the patches of every DMC case, 16 bits per patch (patch k in bits 16k..16k+11),
@@ -1525,15 +1533,18 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
     }
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
-        myslot[8 + h] = ((hbits >> h) & 1) ? nslot + __popc(hb & ((1 << h) - 1)) : -1;
-        if (lane == kWave - 1) L.hslot[h] = (uint8_t)myslot[8 + h];
+        // hb is wave-uniform, so every lane writes the same halo entry (kinds 0, 1, 3, 5)
+        const int hs = ((hb >> h) & 1) ? nslot + __popc(hb & ((1 << h) - 1)) : -1;
+        myslot[8 + h] = ((hbits >> h) & 1) ? hs : -1;
+        L.slot[h < 2 ? h : 2 * h - 1][kWave] = (uint8_t)hs;
     }
     nslot += __popc(hb);
     const bool slots = nslot <= kEdgeSlots;  // wave-uniform
     if (slots) {
+        // slots are < kEdgeSlots here, so the edges that do not cross (-1 & 0xff) all write the
+        // spare last item, and no lane needs its own branch
 #pragma unroll
-        for (int k = 0; k < 12; ++k)
-            if (myslot[k] >= 0) L.eitem[myslot[k]] = (uint16_t)((lane << 4) | k);
+        for (int k = 0; k < 12; ++k) L.eitem[myslot[k] & 0xff] = (uint16_t)((lane << 4) | k);
     }
     wave_lds_sync();
 
@@ -1559,26 +1570,23 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
         const int num = __popc(em);
         S sx = S(0), sy = S(0), sz = S(0);
         if (slots) {
-            // the slots of all twelve edges first (independent reads), then the patch's edges'
-            // vertices, added in ascending edge order
-            const int nx = src + 1 < kWave ? src + 1 : src;
-            const int cap = nslot > 0 ? nslot - 1 : 0;
+            // the slots of all twelve edges first (independent reads), then all twelve vertices
+            // added in ascending edge order, edges outside the patch (and, on NaN samples only,
+            // patch edges that do not cross) reading the zero entry: the running sum starts at
+            // +0 and so is never -0, and adding +0 leaves it unchanged
             int sl[12];
 #pragma unroll
             for (int e = 0; e < 12; ++e) {
                 const int v = (int)(kKinds >> (5 * e)) & 31, kd = v & 7, top = v >> 3;
-                // a top edge belongs to the next position (lane src + 1, or the halo after lane 63)
-                const int sv = !top ? L.slot[kd][src] : (src == kWave - 1 ? L.hslot[(kd + 1) >> 1] : L.slot[kd][nx]);
-                // only a NaN sample can make a patch edge not cross: keep the read inside the table
-                sl[e] = sv < nslot ? sv : cap;
+                // a top edge belongs to the next position (lane src + 1, or the halo entry after lane 63)
+                const int sv = L.slot[kd][src + top];
+                sl[e] = ((em >> e) & 1) ? sv : kEdgeSlots;
             }
 #pragma unroll
             for (int e = 0; e < 12; ++e) {
-                if ((em >> e) & 1) {
-                    sx += L.ev[0][sl[e]];
-                    sy += L.ev[1][sl[e]];
-                    sz += L.ev[2][sl[e]];
-                }
+                sx += L.ev[0][sl[e]];
+                sy += L.ev[1][sl[e]];
+                sz += L.ev[2][sl[e]];
             }
         } else {
             const int z = z0 + src;
@@ -1622,6 +1630,7 @@ __global__ __launch_bounds__(kBlock, 6) void emit_dmc_kernel(In<S> in, const Seg
     __shared__ uint32_t s_take;
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     if (tid == 0) s_take = 0;
+    if (lane < 3) s_lds[wave].ev[lane][kEdgeSlots] = S(0);  // the zero entry of the patch sums
     __syncthreads();
     const Geo &g = in.geo;
     const SegCtx sc{seg, info_of(seg, g), 0, 0};
@@ -2066,6 +2075,7 @@ __global__ __launch_bounds__(kBlock, 6) void emit_dmc_batch_kernel(const BatchGr
     __shared__ uint32_t s_take;
     const int tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
     if (tid == 0) s_take = 0;
+    if (lane < 3) s_lds[wave].ev[lane][kEdgeSlots] = S(0);  // the zero entry of the patch sums
     __syncthreads();
     const int64_t b = xcd_block(blockIdx.x, nblocks);
     const int j = find_group(rf, m, b);
