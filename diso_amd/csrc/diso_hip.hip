@@ -1982,9 +1982,22 @@ __device__ __forceinline__ void edge_grad_one(const void *__restrict__ quads, in
                                               S *__restrict__ ge) {
     const bool compat = (adj.flags & DISO_FLAG_REFERENCE_COMPAT) != 0;
     V3<S> sum = {S(0), S(0), S(0)};
+    // the quad's four indices: one 16-byte non-temporal load when they are 32-bit and aligned (a
+    // stream read once, kept out of the caches the following gather re-reads samples from)
+    int64_t qv[4];
+    if (!(adj.flags & DISO_FLAG_INDEX64) && ((uintptr_t)quads & 15) == 0) {  // uniform
+        const i32x4 r = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(quads) + q);
+        qv[0] = r.x;
+        qv[1] = r.y;
+        qv[2] = r.z;
+        qv[3] = r.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qv[k] = load_index(quads, 4 * q + k, adj.flags);
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int64_t v = load_index(quads, 4 * q + k, adj.flags);
+        const int64_t v = qv[k];
         if (v < 0 || v >= adj.n) continue;
         const int info = vinfo[v];
         const V3<S> a = load_adj(adj, compat ? v - (info >> 3) : v);
