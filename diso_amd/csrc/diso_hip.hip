@@ -1095,11 +1095,19 @@ __device__ __forceinline__ V3<S> epilogue(V3<S> p, int flags, const Norm<S> &nm)
     return p;
 }
 
-__device__ __forceinline__ void store_index(void *__restrict__ faces, int64_t i, int64_t v, int flags) {
-    if (flags & DISO_FLAG_INDEX64)
-        put(reinterpret_cast<int64_t *>(faces) + i, (int64_t)v);
-    else
-        put(reinterpret_cast<int32_t *>(faces) + i, (int32_t)v);
+// Triangle row f = (a, b, c), as three int64 or int32 indices (the int32 row in one 12-byte store).
+typedef int i32x3u __attribute__((ext_vector_type(3), aligned(4)));
+__device__ __forceinline__ void store_tri(void *__restrict__ faces, int64_t f, int64_t a, int64_t b, int64_t c,
+                                          int flags) {
+    if (flags & DISO_FLAG_INDEX64) {
+        int64_t *p = reinterpret_cast<int64_t *>(faces) + 3 * f;
+        put(p + 0, a);
+        put(p + 1, b);
+        put(p + 2, c);
+    } else {
+        const i32x3u v = {(int)a, (int)b, (int)c};
+        __builtin_nontemporal_store(v, reinterpret_cast<i32x3u *>(reinterpret_cast<int32_t *>(faces) + 3 * f));
+    }
 }
 
 // Quad q = base + rel[0..3], as four int64 or int32 indices; `vec`: `faces` is 16-byte aligned,
@@ -1267,14 +1275,16 @@ __device__ __forceinline__ void emit_mc_segment(const In<S> &in, const SegCtx &s
         const V3<S> p = edge_vertex<S, DEF>(in, px, py, z0 + src, a);
         store_vertex(verts, vbase + k, out.vcap, epilogue(p, out.flags, nm));
     }
-    // corner k of the segment's triangles: corner c of triangle k / 3, whose table edge comes from
-    // its cell's case word
-    const int64_t f0 = 3 * tbase, flim = 3 * out.fcap;
-    for (int k = lane; k < 3 * tt; k += kWave) {
-        const int tr = k / 3, c = k - 3 * tr;
-        const int it = L.tri[tr], src = it >> 3;
-        const int e = DISO_MC_EDGE(L.cw[src], 3 * (it & 7) + c);
-        if (f0 + k < flim) store_index(faces, f0 + k, vbase + L.cand[src][e], out.flags);
+    // triangle k of the segment per lane: its cell's case word once, the three corners' table
+    // edges and vertex ids (independent LDS reads), then the three indices as one row
+    for (int k = lane; k < tt; k += kWave) {
+        const int64_t f = tbase + k;
+        if (f >= out.fcap) continue;
+        const int it = L.tri[k], src = it >> 3, t = it & 7;
+        const uint64_t cw = L.cw[src];
+        const int32_t *cand = L.cand[src];
+        store_tri(faces, f, vbase + cand[DISO_MC_EDGE(cw, 3 * t)], vbase + cand[DISO_MC_EDGE(cw, 3 * t + 1)],
+                  vbase + cand[DISO_MC_EDGE(cw, 3 * t + 2)], out.flags);
     }
 }
 
