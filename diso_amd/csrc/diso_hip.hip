@@ -667,6 +667,48 @@ __device__ __forceinline__ bool short_tails(const Geo &g) {
     return g.nchunk >= 2 && g.g2 - kWave * (g.nchunk - 1) <= kWave / 2;
 }
 
+// A count loop's position in one grid: the segment's row and chunk, and the offset of the stored
+// row (px - off, py - off) in samples, kept by additions as the loop advances segment by segment
+// (instead of 64-bit products per segment; meaningful only where the row is stored).
+struct CountPos {
+    SegPos sp;
+    int64_t rowoff;
+};
+__device__ __forceinline__ CountPos count_pos(const Geo &g, int64_t s) {
+    const SegPos sp = seg_pos(g, s);
+    return CountPos{sp, ((int64_t)(sp.px - g.off) * g.n1 + (sp.py - g.off)) * g.n2};
+}
+__device__ __forceinline__ void count_advance(const Geo &g, CountPos &c) {
+    if (++c.sp.chunk == g.nchunk) {
+        c.sp.chunk = 0;
+        c.rowoff += g.n2;
+        if (++c.sp.py == g.g1) {
+            c.sp.py = 0;
+            ++c.sp.px;
+            c.rowoff += (int64_t)(g.n1 - g.g1) * g.n2;
+        }
+    }
+}
+
+// The corner window of a wave's segment at `c` (pz = chunk * 64 + lane): a window of stored
+// samples from the row offset, else load_box's general path.
+template <typename S>
+__device__ __forceinline__ void load_count_window(const In<S> &in, CornerWin<S> &w, const CountPos &c) {
+    if (!w.interior) {  // wave-uniform
+        w.template load_box<0, 2, 0, 2, 0, 2>(in);
+        return;
+    }
+    const Geo &g = in.geo;
+    const int64_t sy = g.n2, sx = (int64_t)g.n1 * g.n2;
+    const S *p = in.sdf + c.rowoff + (w.pz - g.off);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) w.v[i][j][k] = p[i * sx + j * sy + k];
+}
+
 // Counts the segments [wlo, whi) of one grid whose segment 0 is segment `first` of the launch
 // (a short_tails grid): its full segments one per wave iteration, then its last-chunk segments
 // ("tails") several per iteration, P lanes each (P = tail positions rounded up to a power of two),
@@ -680,13 +722,14 @@ __device__ __forceinline__ uint32_t count_range_tails(const In<S> &in, int algo,
     const Geo &g = in.geo;
     const int nc = g.nchunk, last = nc - 1, tailr = g.g2 - kWave * last;
     uint32_t gfl = 0;
-    SegPos sp = seg_pos(g, wlo - first);
+    CountPos cp = count_pos(g, wlo - first);
     for (int64_t s = wlo; s < whi; ++s) {  // wave-uniform
+        const SegPos sp = cp.sp;
         if (sp.chunk != last) {
             int nv, nf;
             uint32_t fl, iw;
             CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
-            w.template load_box<0, 2, 0, 2, 0, 2>(in);
+            load_count_window(in, w, cp);
             count_position<S>(in, algo, cases, w, &nv, &nf, &fl, &iw);
             int tv, tf;
             wave_totals2(nv, nf, &tv, &tf);
@@ -701,13 +744,7 @@ __device__ __forceinline__ uint32_t count_range_tails(const In<S> &in, int algo,
             wf += tf;
             gfl |= fl;
         }
-        if (++sp.chunk == nc) {
-            sp.chunk = 0;
-            if (++sp.py == g.g1) {
-                sp.py = 0;
-                ++sp.px;
-            }
-        }
+        count_advance(g, cp);
     }
     // tails of the range: segments t0, t0 + nc, ... below whi (t0 - first = last mod nc)
     const int64_t t0 = wlo + ((last - (wlo - first) % nc) + nc) % nc;
@@ -782,20 +819,15 @@ __device__ __forceinline__ void count_range(const CountSrc<S> &src, const BatchG
             gfl = count_range_tails<S, WRITE>(in, algo, first, seg, info, cases, s, end, lane, wv, wf);
             s = end;
         }
-        SegPos sp = seg_pos(in.geo, s - first);
+        CountPos cp = count_pos(in.geo, s - first);
         for (; s < end; ++s) {
             int nv, nf;
             uint32_t fl, iw;
+            const SegPos sp = cp.sp;
             CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
-            w.template load_box<0, 2, 0, 2, 0, 2>(in);
+            load_count_window(in, w, cp);
             count_position<S>(in, algo, cases, w, &nv, &nf, &fl, &iw);
-            if (++sp.chunk == in.geo.nchunk) {  // next segment of the grid
-                sp.chunk = 0;
-                if (++sp.py == in.geo.g1) {
-                    sp.py = 0;
-                    ++sp.px;
-                }
-            }
+            count_advance(in.geo, cp);  // next segment of the grid
             int tv, tf;
             wave_totals2(nv, nf, &tv, &tf);
             if constexpr (WRITE) {
