@@ -354,6 +354,9 @@ struct Win {
     S v[NX][NY][NZ];
     int px, py, pz;
     bool interior;
+    // a window whose interior test the caller has made (a count loop keeps it by row)
+    __device__ __forceinline__ Win(int px_, int py_, int pz_, bool interior_)
+        : px(px_), py(py_), pz(pz_), interior(interior_) {}
     __device__ __forceinline__ Win(const In<S> &in, int px_, int py_, int pz_, int z0) : px(px_), py(py_), pz(pz_) {
         const Geo &g = in.geo;
         interior = px + X0 - g.off >= 0 && px + X0 + NX - 1 - g.off < g.n0 && py + Y0 - g.off >= 0 &&
@@ -670,13 +673,25 @@ __device__ __forceinline__ bool short_tails(const Geo &g) {
 // A count loop's position in one grid: the segment's row and chunk, and the offset of the stored
 // row (px - off, py - off) in samples, kept by additions as the loop advances segment by segment
 // (instead of 64-bit products per segment; meaningful only where the row is stored).
+// Whether a segment's corner window is all stored samples is kept the same way: rows (px..px+1,
+// py..py+1) stored, updated when the row changes, and the chunk within [clo, chi], the chunks
+// whose z extent z0 .. z0 + 64 is stored.
 struct CountPos {
     SegPos sp;
     int64_t rowoff;
+    bool rowin;
+    int clo, chi;
+    __device__ __forceinline__ bool interior() const { return rowin && sp.chunk >= clo && sp.chunk <= chi; }
 };
+__device__ __forceinline__ bool rows_stored(const Geo &g, int px, int py) {
+    return px - g.off >= 0 && px + 1 - g.off < g.n0 && py - g.off >= 0 && py + 1 - g.off < g.n1;
+}
 __device__ __forceinline__ CountPos count_pos(const Geo &g, int64_t s) {
     const SegPos sp = seg_pos(g, s);
-    return CountPos{sp, ((int64_t)(sp.px - g.off) * g.n1 + (sp.py - g.off)) * g.n2};
+    // chunk c is inside when 64 c - off >= 0 and 64 c + 64 - off < n2
+    const int clo = (g.off + kWave - 1) / kWave, hi = g.n2 + g.off - kWave - 1;
+    return CountPos{sp, ((int64_t)(sp.px - g.off) * g.n1 + (sp.py - g.off)) * g.n2, rows_stored(g, sp.px, sp.py), clo,
+                    hi >= 0 ? hi / kWave : -1};
 }
 __device__ __forceinline__ void count_advance(const Geo &g, CountPos &c) {
     if (++c.sp.chunk == g.nchunk) {
@@ -687,6 +702,7 @@ __device__ __forceinline__ void count_advance(const Geo &g, CountPos &c) {
             ++c.sp.px;
             c.rowoff += (int64_t)(g.n1 - g.g1) * g.n2;
         }
+        c.rowin = rows_stored(g, c.sp.px, c.sp.py);
     }
 }
 
@@ -728,7 +744,7 @@ __device__ __forceinline__ uint32_t count_range_tails(const In<S> &in, int algo,
         if (sp.chunk != last) {
             int nv, nf;
             uint32_t fl, iw;
-            CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
+            CornerWin<S> w(sp.px, sp.py, sp.chunk * kWave + lane, cp.interior());
             load_count_window(in, w, cp);
             count_position<S>(in, algo, cases, w, &nv, &nf, &fl, &iw);
             int tv, tf;
@@ -824,7 +840,7 @@ __device__ __forceinline__ void count_range(const CountSrc<S> &src, const BatchG
             int nv, nf;
             uint32_t fl, iw;
             const SegPos sp = cp.sp;
-            CornerWin<S> w(in, sp.px, sp.py, sp.chunk * kWave + lane, sp.chunk * kWave);
+            CornerWin<S> w(sp.px, sp.py, sp.chunk * kWave + lane, cp.interior());
             load_count_window(in, w, cp);
             count_position<S>(in, algo, cases, w, &nv, &nf, &fl, &iw);
             count_advance(in.geo, cp);  // next segment of the grid
