@@ -54,9 +54,6 @@ constexpr int kCtlHeader = 64;             // bytes
 
 __constant__ uint64_t kMcCase[256] = {DISO_MC_CASE_VALUES};
 __constant__ uint64_t kDmcCase[256] = {DISO_DMC_CASE_VALUES};
-__constant__ int kEdgeOfs[12][3] = {DISO_EDGE_OFS_VALUES};
-__constant__ int kEdgeAxis[12] = {DISO_EDGE_AXIS_VALUES};
-__constant__ int kQuad[6][4][4] = {DISO_QUAD_VALUES};
 
 // ------------------------------------------------------------------------------------------
 // Geometry and sampling
@@ -310,9 +307,10 @@ struct SegBase {
     int64_t f;  // first triangle (MC) / first DMC vertex (patch) of the segment
 };
 
-__device__ __forceinline__ int64_t seg_index(const Geo &g, int px, int py, int chunk) {
-    return ((int64_t)px * g.g1 + py) * g.nchunk + chunk;
-}
+// Segment (px, py, chunk) is (px * g1 + py) * nchunk + chunk, so the ids step by nchunk from row
+// (px, py) to (px, py + 1) and by xstride(g) to (px + 1, py): neighbour rows' segments come from
+// additions, not products.
+__device__ __forceinline__ int64_t xstride(const Geo &g) { return (int64_t)g.g1 * g.nchunk; }
 
 // Position classification word, written by the count pass for every lane of every segment
 // (segment-major: word s * 64 + lane), read by the later passes instead of re-deriving it from
@@ -1253,9 +1251,8 @@ __device__ __forceinline__ McSegIn mc_load(const In<S> &in, const SegCtx &sc, in
     d.s = s;
     d.sp = sp;
     // This row has cells, so rows +x, +y, +x+y exist; +z words are read within the row only.
-    const int px = sp.px, py = sp.py, chunk = sp.chunk, pz = chunk * kWave + lane;
-    const int64_t s10 = seg_index(g, px + 1, py, chunk), s01 = seg_index(g, px, py + 1, chunk),
-                  s11 = seg_index(g, px + 1, py + 1, chunk);
+    const int pz = sp.chunk * kWave + lane;
+    const int64_t s10 = s + xstride(g), s01 = s + g.nchunk, s11 = s10 + g.nchunk;
     d.vabs = seg[s].v;
     d.fabs = seg[s].f;
     d.d10 = (int32_t)(seg[s10].v - d.vabs);
@@ -1379,6 +1376,23 @@ __device__ constexpr int stencil_eid(int ty, int k) {
     return ty < 3 ? q[ty][k] : q[ty - 3][(4 - k) & 3];
 }
 
+// The edge and quad tables as the kernels evaluate them (eox..eax, stencil_ci / stencil_eid) agree
+// with the tables' entries (diso_tables.h: edge offsets and axes, quad cell offsets and edge ids).
+__device__ constexpr bool tables_agree() {
+    const int ofs[12][3] = {DISO_EDGE_OFS_VALUES};
+    const int axis[12] = {DISO_EDGE_AXIS_VALUES};
+    const int quad[6][4][4] = {DISO_QUAD_VALUES};
+    for (int e = 0; e < 12; ++e)
+        if (eox(e) != ofs[e][0] || eoy(e) != ofs[e][1] || eoz(e) != ofs[e][2] || eax(e) != axis[e]) return false;
+    for (int ty = 0; ty < 6; ++ty)
+        for (int k = 0; k < 4; ++k) {
+            const int *c = quad[ty][k];
+            if (stencil_ci(ty, k) != -4 * c[0] - 2 * c[1] - c[2] || stencil_eid(ty, k) != c[3]) return false;
+        }
+    return true;
+}
+static_assert(tables_agree(), "edge / quad tables");
+
 // Edge kinds of a segment's cells, by owner row (dx, dy) and axis: 0: row 00 x, 1: row 00 y,
 // 2: row 00 z, 3: row 10 y, 4: row 10 z, 5: row 01 x, 6: row 01 z, 7: row 11 z (the owner of an
 // edge is the position it starts from; row dxy is (px + dx, py + dy)). The x and y edges of the
@@ -1493,8 +1507,7 @@ __device__ __forceinline__ DmcSegIn dmc_load(const In<S> &in, const SegCtx &sc, 
     const int px = sp.px, py = sp.py, chunk = sp.chunk, pz = chunk * kWave + lane;
     d.vabs = seg[s].v;
     d.pabs = seg[s].f;
-    const int64_t s10 = seg_index(g, px + 1, py, chunk), s01 = seg_index(g, px, py + 1, chunk),
-                  s11 = seg_index(g, px + 1, py + 1, chunk);
+    const int64_t s10 = s + xstride(g), s01 = s + g.nchunk, s11 = s10 + g.nchunk;
     d.w0 = info[s * kWave + lane];
     d.w10 = info[s10 * kWave + lane];
     d.w01 = info[s01 * kWave + lane];
@@ -1510,7 +1523,7 @@ __device__ __forceinline__ DmcSegIn dmc_load(const In<S> &in, const SegCtx &sc, 
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const bool row = px - i >= 0 && py - j >= 0;  // wave-uniform
-            const int64_t r = row ? seg_index(g, px - i, py - j, chunk) : s;
+            const int64_t r = row ? s - i * xstride(g) - j * g.nchunk : s;
             d.rowd[2 * i + j] = row ? (int32_t)(seg[r].f - d.pabs) : 0;
             d.cur[2 * i + j] = (i || j) ? (row ? info[r * kWave + lane] : 0u) : d.w0;
             d.prev[2 * i + j] = (row && !(i && j) && pz >= 1) ? info[r * kWave + lane - 1] : 0u;
@@ -1525,7 +1538,6 @@ __device__ __forceinline__ void emit_dmc_segment(const In<S> &in, const SegCtx &
                                                  uint8_t *__restrict__ vinfo, const OutCfg &out, const Norm<S> &nm,
                                                  DmcEmitLds<S> &L, int lane) {
     wave_lds_sync();  // the previous segment's LDS reads are done
-    const Geo &g = in.geo;
     const int64_t pabs = d.pabs, pbase = pabs - sc.f0;
     const int px = d.sp.px, py = d.sp.py;
     const int z0 = d.sp.chunk * kWave;
@@ -1845,8 +1857,8 @@ __device__ __forceinline__ bool seg_owns_crossing(const SegBase *__restrict__ se
 // rows -x and -y, and (the z edge into lane 0) by the previous chunk: whether any of them owns a
 // crossing edge.
 __device__ __forceinline__ bool seg_touched(const SegBase *__restrict__ seg, const Geo &g, int64_t s, SegPos sp) {
-    return seg_owns_crossing(seg, s) || (sp.px > 0 && seg_owns_crossing(seg, seg_index(g, sp.px - 1, sp.py, sp.chunk))) ||
-           (sp.py > 0 && seg_owns_crossing(seg, seg_index(g, sp.px, sp.py - 1, sp.chunk))) ||
+    return seg_owns_crossing(seg, s) || (sp.px > 0 && seg_owns_crossing(seg, s - xstride(g))) ||
+           (sp.py > 0 && seg_owns_crossing(seg, s - g.nchunk)) ||
            (sp.chunk > 0 && seg_owns_crossing(seg, s - 1));
 }
 
@@ -1868,8 +1880,8 @@ __device__ __forceinline__ BwdSegIn bwd_load(const Geo &g, const SegCtx &sc, int
     BwdSegIn d;
     d.s = s;
     d.sp = sp;
-    const int64_t sm0 = sp.px > 0 ? seg_index(g, sp.px - 1, sp.py, sp.chunk) : -1;
-    const int64_t s0m = sp.py > 0 ? seg_index(g, sp.px, sp.py - 1, sp.chunk) : -1;
+    const int64_t sm0 = sp.px > 0 ? s - xstride(g) : -1;
+    const int64_t s0m = sp.py > 0 ? s - g.nchunk : -1;
     d.v = seg[s].v;
     d.dm0 = sm0 >= 0 ? (int32_t)(seg[sm0].v - d.v) : 0;
     d.d0m = s0m >= 0 ? (int32_t)(seg[s0m].v - d.v) : 0;
