@@ -491,12 +491,40 @@ __device__ __forceinline__ V3<S> lerp_vertex(int x, int y, int z, int axis, S d0
     return r;
 }
 
+// Samples and deforms of both endpoints of the edge from (x,y,z) along `axis`, as sample() and
+// deform_at() read them: the second endpoint's index is the first's plus the axis stride (one
+// index product per edge instead of two).
+template <typename S, bool DEF>
+struct EdgeEnds {
+    S d0, d1;
+    V3<S> f0, f1;
+};
+template <typename S, bool DEF>
+__device__ __forceinline__ EdgeEnds<S, DEF> edge_ends(const In<S> &in, int x, int y, int z, int axis) {
+    const Geo &g = in.geo;
+    const bool ok0 = in_data(in, x, y, z), ok1 = in_data(in, x + (axis == 0), y + (axis == 1), z + (axis == 2));
+    const int64_t i0 = data_index(in, x, y, z);
+    const int64_t i1 = i0 + (axis == 0 ? (int64_t)g.n1 * g.n2 : (axis == 1 ? (int64_t)g.n2 : 1));
+    const int64_t j0 = ok0 ? i0 : 0, j1 = ok1 ? i1 : 0;
+    EdgeEnds<S, DEF> e;
+    const S a = in.sdf[j0], b = in.sdf[j1];
+    e.d0 = ok0 ? a : in.pad;
+    e.d1 = ok1 ? b : in.pad;
+    e.f0 = e.f1 = V3<S>{S(0), S(0), S(0)};
+    if (DEF) {
+        const S *p0 = in.deform + 3 * j0, *p1 = in.deform + 3 * j1;
+        const S u0 = p0[0], u1 = p0[1], u2 = p0[2], w0 = p1[0], w1 = p1[1], w2 = p1[2];
+        e.f0 = V3<S>{ok0 ? u0 : S(0), ok0 ? u1 : S(0), ok0 ? u2 : S(0)};
+        e.f1 = V3<S>{ok1 ? w0 : S(0), ok1 ? w1 : S(0), ok1 ? w2 : S(0)};
+    }
+    return e;
+}
+
 // MC vertex on the edge from (x,y,z) along `axis`: samples and deforms of both endpoints.
 template <typename S, bool DEF>
 __device__ __forceinline__ V3<S> edge_vertex(const In<S> &in, int x, int y, int z, int axis) {
-    const int x1 = x + (axis == 0), y1 = y + (axis == 1), z1 = z + (axis == 2);
-    return lerp_vertex<S, DEF>(x, y, z, axis, sample(in, x, y, z), sample(in, x1, y1, z1),
-                               deform_at<S, DEF>(in, x, y, z), deform_at<S, DEF>(in, x1, y1, z1), in.iso);
+    const EdgeEnds<S, DEF> e = edge_ends<S, DEF>(in, x, y, z, axis);
+    return lerp_vertex<S, DEF>(x, y, z, axis, e.d0, e.d1, e.f0, e.f1, in.iso);
 }
 
 // x / c for a divisor c fixed per call, with r = RN(1/c): q = x*r is within an ulp of x/c and
@@ -1965,10 +1993,9 @@ __device__ __forceinline__ void bwd_segment(const In<S> &in, const SegCtx &sc, c
         const int axis = slot % 3, end = slot >= 3;
         // the edge runs from (x, y, z) along `axis`; the vertex is its start (slots 0-2) or its end
         const int x = px - (slot == 3), y = py - (slot == 4), z = z0 + src - (slot == 5);
-        const int ex = x + (axis == 0), ey = y + (axis == 1), ez = z + (axis == 2);
-        const S d0 = sample(in, x, y, z), d1 = sample(in, ex, ey, ez);
-        const V3<S> f0 = deform_at<S, DEF>(in, x, y, z), f1 = deform_at<S, DEF>(in, ex, ey, ez);
-        return edge_adjoint<S, DEF>(x, y, z, axis, end, d0, d1, f0, f1, iso, load_adj(adj, vbase + L.id[slot][src]));
+        const EdgeEnds<S, DEF> e = edge_ends<S, DEF>(in, x, y, z, axis);
+        return edge_adjoint<S, DEF>(x, y, z, axis, end, e.d0, e.d1, e.f0, e.f1, iso,
+                                    load_adj(adj, vbase + L.id[slot][src]));
     };
     // Rounds of up to 128 items: with more than 64 left, each lane takes two (the second clamped
     // to the last item, its result unused), so both gathers are in flight together.
