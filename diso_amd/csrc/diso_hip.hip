@@ -1250,7 +1250,8 @@ __device__ __forceinline__ void edge_ids(int32_t *c, int32_t r00, int32_t r10, i
 }
 
 // MC emission. Vertex item = (lane, axis) of an owned crossing edge, in the reference's order
-// (cell, then axis x < y < z; src/cumc.cu:370-410); triangle corner item = (lane, table edge).
+// (cell, then axis x < y < z; src/cumc.cu:370-410); triangle item = (lane, triangle of its cell),
+// whose three corners' table edges come from the cell's case word.
 // Cell codes and crossing masks come from the count pass's classification words.
 struct McEmitLds {
     int32_t cand[kWave][12];  // vertex id of each table edge, relative to the segment's first vertex
